@@ -1,0 +1,126 @@
+/* MI355X extensions of libtprims: HIP executors, plan configuration
+ * (complex method, split-BF16 / FP8 precision, tile override), plan reports,
+ * grouped execution of many products in one persistent kernel, and the
+ * vector-Jacobian product of a contraction.
+ *
+ * None of these has a C counterpart in the reference except where cited; the
+ * cited item is the Rust interface the entry point stands in for. Streams
+ * are passed as `void*` (a hipStream_t) so that a C consumer needs no HIP
+ * headers. */
+#ifndef TPRIMS_HIP_H
+#define TPRIMS_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+#include "tprims/core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- executors ------------------------------------------------------------
+ * Counterpart of crates/tprims-exec/src/exec.rs:41 (Exec) and the executor
+ * handles of crates/tprims-capi/src/executor.rs:278: an executor owns the
+ * device scratch arena and the stream the contraction kernels run on. */
+
+/* An executor on `device` that runs on `stream` (a hipStream_t, borrowed, not
+ * destroyed), or on a stream of its own when `stream` is NULL. */
+TAPP_EXPORT TAPP_error tprims_tapp_executor_create_hip(TAPP_executor* out, int device, void* stream);
+/* async != 0: TAPP_execute_product returns once the work is enqueued, when every
+ * operand is device memory (host operands always synchronize). Default 0. */
+TAPP_EXPORT TAPP_error tprims_tapp_executor_set_async(TAPP_executor exec, int async);
+TAPP_EXPORT TAPP_error tprims_tapp_executor_synchronize(TAPP_executor exec);
+TAPP_EXPORT TAPP_error tprims_tapp_executor_get_stream(TAPP_executor exec, void** stream);
+TAPP_EXPORT TAPP_error tprims_tapp_executor_get_device(TAPP_executor exec, int* device);
+
+/* ---- plan configuration (crates/tprims-contract/src/plan/config.rs:101) ---- */
+enum {
+  TPRIMS_CPLX_NATIVE = 0, /* four real MFMA products per complex block */
+  TPRIMS_CPLX_1M = 1,     /* one real product over 1e/1r-expanded operands */
+  TPRIMS_CPLX_3M = 2      /* three products, Karatsuba recombination */
+};
+enum {
+  TPRIMS_PREC_STORAGE = 0,      /* f32 MFMA for f32/c32, f64 MFMA for f64/c64 */
+  TPRIMS_PREC_SPLIT_BF16_2 = 1, /* f32/c32: hi*hi + hi*lo + lo*hi on bf16 MFMA */
+  TPRIMS_PREC_SPLIT_BF16_3 = 2, /* f32/c32: six bf16 products, ~fp32 accuracy */
+  TPRIMS_PREC_FP8_E4M3 = 3      /* f32/c32: e4m3 operands with per-panel scales */
+};
+typedef struct {
+  int32_t complex_method; /* TPRIMS_CPLX_* */
+  int32_t precision;      /* TPRIMS_PREC_* */
+  int32_t tile;           /* 0: chosen by the planner's tuner; else a tile id (see report) */
+  int32_t reserved[5];
+} tprims_plan_config;
+
+/* TAPP_create_tensor_product with a configuration (NULL = defaults). A prec of
+ * TAPP_BF16BF16_ACCUM_F32 selects TPRIMS_PREC_SPLIT_BF16_3 unless cfg says
+ * otherwise. */
+TAPP_EXPORT TAPP_error tprims_create_tensor_product_ex(TAPP_tensor_product* plan, TAPP_handle handle,
+                                                       TAPP_element_op op_A, TAPP_tensor_info A, const int64_t* idx_A,
+                                                       TAPP_element_op op_B, TAPP_tensor_info B, const int64_t* idx_B,
+                                                       TAPP_element_op op_C, TAPP_tensor_info C, const int64_t* idx_C,
+                                                       TAPP_element_op op_D, TAPP_tensor_info D, const int64_t* idx_D,
+                                                       TAPP_prectype prec, const tprims_plan_config* cfg);
+
+/* Execute with borrowed DLPack operands (the operand contract of
+ * crates/tprims-capi/src/tensor.rs:140-230, extended to ROCm memory). Each
+ * view's shape and strides (NULL strides: compact row-major) must equal the
+ * extents and strides of the info the plan was built from, its dtype the
+ * plan's, lanes 1; devices kDLROCM (the executor's device), kDLROCMHost, kDLCPU
+ * or kDLCUDAHost. A D flagged read-only is TPRIMS_ERR_READ_ONLY. C.view may be
+ * NULL only with a zero beta. */
+TAPP_EXPORT TAPP_error tprims_execute_product_dlpack(TAPP_tensor_product plan, TAPP_executor exec,
+                                                     const void* alpha, tprims_tensor A, tprims_tensor B,
+                                                     const void* beta, tprims_tensor C, tprims_tensor D);
+
+/* What the planner concluded (crates/tprims-contract/src/plan/report.rs:33). */
+typedef struct {
+  int64_t m, n, k, batch;   /* reduced matrix view after folding */
+  int64_t flops;            /* 2 (real) or 8 (complex) per multiply-add */
+  int32_t dtype;            /* TAPP_F32 .. TAPP_C64 */
+  int32_t complex_method;
+  int32_t precision;
+  int32_t tile;             /* tile id used */
+  int32_t tile_m, tile_n, tile_k;
+  int32_t a_k_major, b_k_major; /* operand gathers run along K (1) or along M/N (0) */
+  int32_t d_m_major;        /* write-back runs along M (1) or N (0) */
+  int32_t m_axes, n_axes, k_axes, h_axes; /* folded axis counts */
+  int32_t strategy;         /* 0 contraction kernel, 1 output pass only, 2 empty */
+  char kernel[64];
+} tprims_plan_report;
+TAPP_EXPORT TAPP_error tprims_product_report(TAPP_tensor_product plan, tprims_plan_report* out);
+
+/* ---- grouped execution (crates/tprims-contract/src/batch.rs:125) -----------
+ * Many products of one dtype (each its own plan and shape) bound to their
+ * operands and run as one persistent kernel over a device work list. alpha and
+ * beta arrays hold one scalar per item (of the dtype). All pointers are device
+ * memory. The group owns its work list; create_graph captures the launch into a
+ * hipGraph that execute then replays. */
+typedef intptr_t tprims_group;
+TAPP_EXPORT TAPP_error tprims_group_create(tprims_group* out, TAPP_executor exec, int num_items,
+                                           const TAPP_tensor_product* plans, const void* alpha,
+                                           const void* const* A, const void* const* B, const void* beta,
+                                           const void* const* C, void* const* D);
+TAPP_EXPORT TAPP_error tprims_group_execute(tprims_group group, TAPP_executor exec);
+TAPP_EXPORT TAPP_error tprims_group_use_graph(tprims_group group, TAPP_executor exec, int enable);
+/* Launch every item separately (one kernel per item), for comparison. */
+TAPP_EXPORT TAPP_error tprims_group_execute_each(tprims_group group, TAPP_executor exec);
+TAPP_EXPORT TAPP_error tprims_group_info(tprims_group group, int64_t* tiles, int64_t* flops, int32_t* tile_id);
+TAPP_EXPORT TAPP_error tprims_group_destroy(tprims_group group);
+
+/* ---- differentiation --------------------------------------------------------
+ * For D = op_D(alpha * op_A(A) * op_B(B) + beta * op_C(C)) and a cotangent dD
+ * (in D's layout), writes the cotangents of A and B in A's and B's layouts,
+ * in the convention of conjugate Wirtinger derivatives (PyTorch's):
+ *   dA = op_A(conj(alpha) * op_D(dD) * conj(op_B(B)))  summed over the labels not in A,
+ * and symmetrically dB. A repeated label of A writes only A's diagonal (the
+ * other addressed elements are zeroed); a label of A absent from B and D
+ * receives the same value at every index (broadcast). accumulate != 0 adds into
+ * dA/dB instead of overwriting. dA or dB may be NULL to skip it. */
+TAPP_EXPORT TAPP_error tprims_product_vjp(TAPP_tensor_product plan, TAPP_executor exec, const void* alpha,
+                                          const void* A, const void* B, const void* dD, void* dA, void* dB,
+                                          int accumulate);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

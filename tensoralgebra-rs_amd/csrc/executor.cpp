@@ -1,0 +1,261 @@
+// Executor runtime: stream and scratch ownership, the pre-write validation
+// of an execution, host-operand staging and the launch of the planned kernel.
+#include "executor.h"
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace tp {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fail(ERR_INTERNAL, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+DeviceGuard::DeviceGuard(int d) {
+  int cur = 0;
+  if (hipGetDevice(&cur) != hipSuccess) fail(ERR_DEVICE, "no HIP device is available");
+  if (cur != d) {
+    hip_check(hipSetDevice(d), "hipSetDevice");
+    prev = cur;
+  }
+}
+DeviceGuard::~DeviceGuard() {
+  if (prev >= 0) (void)hipSetDevice(prev);
+}
+
+hipStream_t Executor::get_stream() {
+  std::lock_guard<std::mutex> g(mu);
+  if (!stream_ready) {
+    if (owns_stream) {
+      DeviceGuard dg(device);
+      hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    stream_ready = true;
+  }
+  return stream;
+}
+
+char* Executor::ensure_scratch(size_t bytes) {
+  if (bytes <= scratch_cap) return scratch;
+  if (scratch) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    hip_check(hipFree(scratch), "hipFree");
+    scratch = nullptr;
+    scratch_cap = 0;
+  }
+  size_t cap = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+  hip_check(hipMalloc((void**)&scratch, cap), "hipMalloc(scratch)");
+  scratch_cap = cap;
+  return scratch;
+}
+
+Executor::~Executor() {
+  if (stream_ready && stream) (void)hipStreamSynchronize(stream);
+  if (scratch) (void)hipFree(scratch);
+  if (owns_stream && stream_ready && stream) (void)hipStreamDestroy(stream);
+}
+
+Executor* new_executor(int device, hipStream_t stream) {
+  Executor* e = new Executor;
+  e->device = device;
+  e->stream = stream;
+  e->owns_stream = stream == nullptr;
+  e->stream_ready = stream != nullptr;
+  return e;
+}
+
+Executor* resolve_executor(intptr_t handle) {
+  if (handle != 0) return reinterpret_cast<Executor*>(handle);
+  static std::mutex mu;
+  static std::map<int, Executor*> defaults;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    fail(ERR_DEVICE, "no HIP device is available");
+  }
+  std::lock_guard<std::mutex> g(mu);
+  auto it = defaults.find(dev);
+  if (it != defaults.end()) return it->second;
+  Executor* e = new Executor;  // lives until the process ends
+  e->device = dev;
+  e->stream = nullptr;  // the null stream
+  e->owns_stream = false;
+  e->stream_ready = true;
+  e->is_default = true;
+  defaults[dev] = e;
+  return e;
+}
+
+namespace {
+
+struct Range {
+  bool present = false;
+  __int128 lo = 0, hi = 0;  // byte addresses [lo, hi)
+};
+
+Range byte_range(const Plan& p, int op, const void* origin) {
+  Range r;
+  if (!origin) return r;
+  const Span& s = p.prob.spans[op];
+  if (!s.present) return r;
+  int64_t off = 0;
+  switch (op) {
+    case OP_A: off = p.prob.a.layout.offset; break;
+    case OP_B: off = p.prob.b.layout.offset; break;
+    case OP_C: off = p.prob.c.layout.offset; break;
+    case OP_D: off = p.prob.d.layout.offset; break;
+  }
+  const __int128 base = (__int128)(uintptr_t)origin, es = elem_size(p.prob.dtype);
+  r.present = true;
+  r.lo = base + ((__int128)s.lo - off) * es;
+  r.hi = base + ((__int128)s.hi - off + 1) * es;
+  return r;
+}
+
+bool overlap(const Range& x, const Range& y) { return x.present && y.present && x.lo < y.hi && y.lo < x.hi; }
+
+// Where an operand lives, as seen from the executor's device.
+struct Where {
+  const void* dev = nullptr;  // usable device address of the origin
+  bool host = false;          // pageable host memory: staged
+};
+
+Where classify(const void* p, int device) {
+  Where w;
+  if (!p) return w;
+  hipPointerAttribute_t at{};
+  hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    w.host = true;
+    return w;
+  }
+  switch (at.type) {
+    case hipMemoryTypeDevice:
+    case hipMemoryTypeManaged:
+    case hipMemoryTypeArray:
+    case hipMemoryTypeUnified:
+      if (at.type == hipMemoryTypeDevice && at.device != device)
+        fail(ERR_DEVICE, "an operand lives on device " + std::to_string(at.device) + ", the executor on device " +
+                             std::to_string(device));
+      w.dev = p;
+      return w;
+    case hipMemoryTypeHost:
+      w.dev = at.devicePointer ? at.devicePointer : p;
+      return w;
+    default:
+      w.host = true;
+      return w;
+  }
+}
+
+bool is_zero(const double v[2]) { return v[0] == 0.0 && v[1] == 0.0; }
+
+}  // namespace
+
+void validate_item(const Plan& p, const Item& it, bool beta_zero) {
+  if (!it.a || !it.b || !it.d) fail(ERR_INVALID_ARGUMENT, "A, B or D data pointer is null or zero");
+  if (!it.c && !beta_zero)
+    fail(ERR_UNSUPPORTED,
+         "a null C (TAPP_IN_PLACE) with a nonzero beta is ambiguous in TAPP and not supported; pass D as C to "
+         "accumulate in place");
+  if (p.prob.out_empty) return;
+  Range rd = byte_range(p, OP_D, it.d);
+  if (overlap(rd, byte_range(p, OP_A, it.a))) fail(ERR_ALIASED, "the output overlaps operand A");
+  if (overlap(rd, byte_range(p, OP_B, it.b))) fail(ERR_ALIASED, "the output overlaps operand B");
+  if (p.prob.c_separate && it.c) {
+    bool same = it.c == it.d && p.prob.c_matches_d;
+    if (!same && overlap(rd, byte_range(p, OP_C, it.c)))
+      fail(ERR_ALIASED, "C and D overlap without being the same mapping at the same origin");
+  }
+}
+
+void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in[2], const Item& it) {
+  if (p.strategy == 2) return;
+  double beta[2] = {beta_in[0], beta_in[1]};
+  if (!it.c) beta[0] = beta[1] = 0.0;
+  const bool read_c = !is_zero(beta);
+  const bool product = p.strategy == 0 && !is_zero(alpha);
+  DeviceGuard dg(ex.device);
+  hipStream_t s = ex.get_stream();
+  const int64_t es = elem_size(p.prob.dtype);
+
+  Where wa, wb, wc, wd;
+  if (product) {
+    wa = classify(it.a, ex.device);
+    wb = classify(it.b, ex.device);
+  }
+  if (read_c) wc = classify(it.c, ex.device);
+  wd = classify(it.d, ex.device);
+
+  const bool staged = wa.host || wb.host || wc.host || wd.host;
+  std::unique_lock<std::mutex> lock(ex.mu, std::defer_lock);
+  struct Stage {
+    int op;
+    const void* host;
+    size_t bytes;
+    int64_t lo;
+    size_t at;
+  };
+  std::vector<Stage> stages;
+  if (staged) {
+    lock.lock();
+    size_t total = 0;
+    auto plan_stage = [&](int op, const void* host) {
+      const Span& sp = p.prob.spans[op];
+      if (!sp.present) return;
+      size_t bytes = (size_t)(sp.hi - sp.lo + 1) * es;
+      stages.push_back({op, host, bytes, sp.lo, total});
+      total += (bytes + 255) & ~(size_t)255;
+    };
+    if (wa.host) plan_stage(OP_A, it.a);
+    if (wb.host) plan_stage(OP_B, it.b);
+    const bool c_is_d = it.c == it.d;
+    if (wc.host && !c_is_d) plan_stage(OP_C, it.c);
+    if (wd.host) plan_stage(OP_D, it.d);
+    char* base = ex.ensure_scratch(total);
+    for (auto& st : stages) {
+      char* dst = base + st.at;
+      const char* src = (const char*)st.host + st.lo * es;
+      hip_check(hipMemcpyAsync(dst, src, st.bytes, hipMemcpyHostToDevice, s), "staging copy to the device");
+      const void* origin = dst - st.lo * es;
+      switch (st.op) {
+        case OP_A: wa.dev = origin; break;
+        case OP_B: wb.dev = origin; break;
+        case OP_C: wc.dev = origin; break;
+        case OP_D: wd.dev = origin; break;
+      }
+    }
+    if (wc.host && c_is_d) wc.dev = wd.dev;
+  }
+
+  ContractArgs a = p.args(ex.device, wa.dev, wb.dev, wc.dev, const_cast<void*>(wd.dev), alpha, beta, read_c);
+  if (!product) {
+    hip_check(launch_output_pass(a, p.prob.dtype, s), "output-pass launch");
+  } else if (p.cfg.precision == PREC_SPLIT_BF16_2 || p.cfg.precision == PREC_SPLIT_BF16_3) {
+    hip_check(launch_contract_split(a, p.prob.dtype, p.cfg.precision == PREC_SPLIT_BF16_2 ? 2 : 3, p.tile, s),
+              "split-BF16 contraction launch");
+  } else {
+    hip_check(launch_contract(a, p.prob.dtype, p.cfg.complex_method, p.tile, s), "contraction launch");
+  }
+
+  if (staged) {
+    for (auto& st : stages)
+      if (st.op == OP_D) {
+        char* dst = (char*)st.host + st.lo * es;
+        hip_check(hipMemcpyAsync(dst, (const char*)wd.dev + st.lo * es, st.bytes, hipMemcpyDeviceToHost, s),
+                  "staging copy to the host");
+      }
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  } else if (!ex.async.load()) {
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  }
+}
+
+}  // namespace tp

@@ -1,0 +1,61 @@
+// The GPU executor: the MI355X counterpart of crates/tprims-exec (Exec, the
+// workspace provider) and of the executor handles of
+// crates/tprims-capi/src/executor.rs. It owns a stream, a device scratch arena
+// (staging of host operands) and the in-flight count that makes destruction
+// refuse with TPRIMS_BUSY while calls run.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <mutex>
+
+#include "plan.h"
+
+namespace tp {
+
+struct Executor {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool owns_stream = false;  // create our own stream lazily
+  bool stream_ready = false;
+  bool is_default = false;
+  std::atomic<int> async{0};
+  size_t pool_size = 0, budget = 1;  // recorded reference thread settings
+  std::atomic<int> inflight{0};
+  std::mutex mu;  // stream creation and scratch
+  char* scratch = nullptr;
+  size_t scratch_cap = 0;
+
+  hipStream_t get_stream();
+  char* ensure_scratch(size_t bytes);  // caller holds mu
+  ~Executor();
+};
+
+// Zero selects the default executor of the current device (null stream).
+Executor* resolve_executor(intptr_t handle);
+Executor* new_executor(int device, hipStream_t stream);
+
+struct Item {
+  const void* a;
+  const void* b;
+  const void* c;
+  void* d;
+};
+
+// The null rules of the C ABI (crates/tprims-capi/src/execute.rs:30-47) and the
+// plan's alias preflight (crates/tprims-contract/src/plan/mod.rs:384-428).
+void validate_item(const Plan& p, const Item& it, bool beta_zero);
+
+// Runs one validated item (plan/mod.rs:487-530 semantics).
+void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta[2], const Item& it);
+
+void hip_check(hipError_t e, const char* what);
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int d);
+  ~DeviceGuard();
+};
+
+}  // namespace tp

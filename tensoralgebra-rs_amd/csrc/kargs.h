@@ -1,0 +1,51 @@
+// Kernel-argument records shared by the host planner and the HIP kernels.
+// Everything is plain data: device pointers, extents and flags.
+#pragma once
+#include <cstdint>
+
+namespace tp {
+
+// One contraction as a batched matrix product over scatter vectors
+// (the GPU form of crates/tprims-contract/src/plan/analysis.rs PackedPlan):
+//   D[hd[h] + dm[m] + dn[n] + dr[r]] =
+//     op_D(alpha * sum_k op_A(A[ha[h] + am[m] + ak[k]]) * op_B(B[hb[h] + bk[k] + bn[n]])
+//          + beta * op_C(C[hc[h] + cm[m] + cn[n]]))      for every r in [0, R)
+// Offsets are in elements of the storage type (complex elements for c32/c64).
+struct ContractArgs {
+  const void* A;
+  const void* B;
+  const void* C;
+  void* D;
+  const int64_t* am;
+  const int64_t* ak;
+  const int64_t* bk;
+  const int64_t* bn;
+  const int64_t* cm;
+  const int64_t* cn;
+  const int64_t* dm;
+  const int64_t* dn;
+  const int64_t* ha;
+  const int64_t* hb;
+  const int64_t* hc;
+  const int64_t* hd;
+  const int64_t* dr;
+  int64_t M, N, K, H, R;
+  double alpha[2];
+  double beta[2];
+  int32_t conj_a, conj_b, conj_c, conj_d;
+  int32_t a_kmajor;  // A gathers run along K (1) or along M (0)
+  int32_t b_kmajor;  // B gathers run along K (1) or along N (0)
+  int32_t d_mmajor;  // the write-back runs along M (1) or along N (0)
+  int32_t read_c;    // beta != 0: C is read (never read otherwise)
+  int32_t tiles_m, tiles_n;
+  int32_t pad0, pad1;
+};
+
+// One item of a grouped launch: the contraction plus where its tiles start in
+// the group's flattened tile space.
+struct GroupItem {
+  ContractArgs args;
+  int64_t tile_begin;  // first global tile index of this item
+};
+
+}  // namespace tp

@@ -1,0 +1,47 @@
+// Host-side entry points of the HIP kernels (implemented in *.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "kargs.h"
+
+namespace tp {
+
+// A workgroup tile of the contraction kernels.
+struct TileCfg {
+  int id;       // stable id (plan reports, configuration override)
+  int bm, bn, bk;
+  int threads;
+  const char* name;
+};
+
+// Tile ids of the storage-precision kernels (16x16x4 MFMA family).
+enum TileId : int {
+  TILE_64x64 = 1,  // 4 waves of 32x32
+  TILE_32x32 = 2,  // 1 wave of 32x32
+  TILE_128x64 = 3, // 4 waves of 64x32
+  TILE_64x32 = 4,  // 2 waves of 32x32
+};
+// Tile ids of the split-BF16 kernels (16x16x32 bf16 MFMA family).
+enum SplitTileId : int {
+  STILE_128x128 = 11,  // 4 waves of 64x64
+  STILE_64x64 = 12,    // 4 waves of 32x32
+};
+
+const TileCfg* tile_cfg(int id);
+
+// The contraction kernel of a storage dtype, complex method and tile.
+// Returns hipSuccess or the launch error.
+hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s);
+// Split-BF16 kernels for f32/c32 storage (nterms 2 or 3).
+hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int tile_id, hipStream_t s);
+// D = op_D(beta * op_C(C)) over the output (empty K or alpha == 0).
+hipError_t launch_output_pass(const ContractArgs& a, DType t, hipStream_t s);
+// Zero every element a layout addresses (rank <= 8; used by the adjoints).
+hipError_t launch_zero_layout(void* base, DType t, int rank, const int64_t* dims, const int64_t* strides,
+                              hipStream_t s);
+// Persistent grouped kernel over `items` (device array) covering `total_tiles`.
+hipError_t launch_grouped(const GroupItem* items, int num_items, int64_t total_tiles, DType t, int method,
+                          int tile_id, int grid, hipStream_t s);
+
+}  // namespace tp

@@ -1,0 +1,363 @@
+// GPU planning: the index analysis of crates/tprims-contract/src/plan/analysis.rs
+// (drop extent-1 axes, sort each role, fold compatible axes, build scatter
+// vectors) plus the decisions that replace the CPU orientation, row-block and
+// blocking rules (plan/orientation.rs:196,307; abi/types.rs:356):
+//  - which axis each operand's gathers run along (coalescing in HBM),
+//  - the K order, blocked so that one K slab reads whole cache lines of both
+//    operands when their contiguous axes are different K axes,
+//  - the workgroup tile.
+#include "plan.h"
+
+#include <algorithm>
+#include <climits>
+#include <memory>
+
+#include "kernels.h"
+
+namespace tp {
+
+std::vector<int64_t> build_scatter(const std::vector<int64_t>& extents, const std::vector<int64_t>& strides) {
+  int64_t total = 1;
+  for (auto e : extents) total *= e;
+  std::vector<int64_t> out;
+  if (total <= 0) return out;
+  out.reserve((size_t)total);
+  const size_t n = extents.size();
+  std::vector<int64_t> counter(n, 0);
+  int64_t offset = 0;
+  for (int64_t t = 0; t < total; t++) {
+    out.push_back(offset);
+    for (size_t d = 0; d < n; d++) {
+      counter[d]++;
+      offset += strides[d];
+      if (counter[d] < extents[d]) break;
+      offset -= strides[d] * extents[d];
+      counter[d] = 0;
+    }
+  }
+  return out;
+}
+
+std::vector<Axis> fold_axes(const std::vector<Axis>& axes) {
+  std::vector<Axis> out;
+  for (const Axis& ax : axes) {
+    if (!out.empty()) {
+      Axis& p = out.back();
+      if (ax.sa == p.sa * p.extent && ax.sb == p.sb * p.extent && ax.sc == p.sc * p.extent &&
+          ax.sd == p.sd * p.extent) {
+        p.extent *= ax.extent;
+        continue;
+      }
+    }
+    out.push_back(ax);
+  }
+  return out;
+}
+
+namespace {
+
+uint64_t uabs(int64_t x) { return x < 0 ? (uint64_t)(-(x + 1)) + 1 : (uint64_t)x; }
+
+// analysis.rs:197-207: extent-1 axes dropped, stable sort by key, fold.
+template <class Key>
+std::vector<Axis> role_axes(const std::vector<RoleAxis>& role, Key key) {
+  std::vector<Axis> v;
+  for (auto& r : role)
+    if (r.extent != 1) v.push_back({r.extent, r.s[OP_A], r.s[OP_B], r.s[OP_C], r.s[OP_D]});
+  std::stable_sort(v.begin(), v.end(), [&](const Axis& x, const Axis& y) { return key(x) < key(y); });
+  return fold_axes(v);
+}
+
+int64_t product(const std::vector<Axis>& axes) {
+  __int128 p = 1;
+  for (auto& a : axes) {
+    p *= a.extent;
+    if (p * 8 > (__int128)LLONG_MAX) fail(ERR_SHAPE, "overflow: role extent product");
+  }
+  return (int64_t)p;
+}
+
+template <class Pick>
+std::vector<int64_t> scatter_for(const std::vector<Axis>& axes, Pick pick) {
+  std::vector<int64_t> e, s;
+  for (auto& a : axes) {
+    e.push_back(a.extent);
+    s.push_back(pick(a));
+  }
+  return build_scatter(e, s);
+}
+
+// Coalescing cost of a gather direction: the element stride between lanes,
+// penalised when the run along it is too short to fill a cache line, and
+// prohibitive for a zero stride (a broadcast gives no contiguity).
+double cost(int64_t stride, int64_t run) {
+  if (stride == 0) return 1e18;
+  double s = (double)uabs(stride);
+  return run >= 4 ? s : s * 8.0;
+}
+
+int64_t largest_divisor_le(int64_t n, int64_t cap) {
+  if (cap < 1) cap = 1;
+  for (int64_t d = std::min(n, cap); d >= 1; d--)
+    if (n % d == 0) return d;
+  return 1;
+}
+
+void push_nonunit(std::vector<Axis>& v, const Axis& a) {
+  if (a.extent > 1) v.push_back(a);
+}
+
+Axis scaled(const Axis& a, int64_t extent, int64_t by) {
+  return {extent, a.sa * by, a.sb * by, a.sc * by, a.sd * by};
+}
+
+bool tile_allowed(int id, DType t, int prec) {
+  if (prec != PREC_STORAGE) return id == STILE_128x128 || id == STILE_64x64;
+  if (id >= 10) return false;
+  if (id == TILE_128x64 && t == DType::C64) return false;  // staging would exceed 64 KiB
+  return tile_cfg(id) != nullptr;
+}
+
+int choose_tile(const Plan& p) {
+  const int prec = p.cfg.precision;
+  if (p.cfg.tile != 0) {
+    if (!tile_allowed(p.cfg.tile, p.prob.dtype, prec)) fail(ERR_UNSUPPORTED, "tile id not available for this plan");
+    return p.cfg.tile;
+  }
+  std::vector<std::pair<int, double>> cands;
+  if (prec != PREC_STORAGE) {
+    cands = {{STILE_128x128, 1.15}, {STILE_64x64, 1.0}};
+  } else {
+    cands = {{TILE_128x64, 1.1}, {TILE_64x64, 1.0}, {TILE_64x32, 0.85}, {TILE_32x32, 0.7}};
+  }
+  int best = 0;
+  double best_score = -1;
+  for (auto [id, eff] : cands) {
+    if (!tile_allowed(id, p.prob.dtype, prec)) continue;
+    const TileCfg* t = tile_cfg(id);
+    double wgs = (double)((p.M + t->bm - 1) / t->bm) * (double)((p.N + t->bn - 1) / t->bn) * (double)p.H;
+    double waves = wgs * (t->threads / 64);
+    double score = std::min(waves, 2048.0) * eff;
+    if (score > best_score + 1e-9) {
+      best_score = score;
+      best = id;
+    }
+  }
+  return best;
+}
+
+}  // namespace
+
+void configure(Plan* plan, const PlanConfig& cfg) {
+  Plan& p = *plan;
+  const Problem& pr = p.prob;
+  p.cfg = cfg;
+  if (cfg.precision != PREC_STORAGE && is_double(pr.dtype))
+    fail(ERR_UNSUPPORTED, "split-BF16 / FP8 precision applies to f32 and c32 only");
+  if (cfg.precision == PREC_FP8) fail(ERR_UNSUPPORTED, "the FP8 path is not available in this build");
+  if (cfg.complex_method < CPLX_NATIVE || cfg.complex_method > CPLX_3M)
+    fail(ERR_INVALID_ARGUMENT, "unknown complex method");
+  if (cfg.precision != PREC_STORAGE && cfg.complex_method != CPLX_NATIVE)
+    fail(ERR_UNSUPPORTED, "split-BF16 runs the native complex schedule only");
+
+  p.tile = choose_tile(p);
+  const TileCfg* t = tile_cfg(p.tile);
+  const int64_t BK = t->bk;
+  const int64_t es = elem_size(pr.dtype);
+
+  // Gather directions.
+  const double inf = 1e30;
+  double a_m = p.m_ax.empty() ? inf : cost(p.m_ax[0].sa, p.m_ax[0].extent);
+  double b_n = p.n_ax.empty() ? inf : cost(p.n_ax[0].sb, p.n_ax[0].extent);
+  int kA = -1, kB = -1;
+  double a_k = inf, b_k = inf;
+  for (size_t i = 0; i < p.k_ax.size(); i++) {
+    double ca = cost(p.k_ax[i].sa, p.k_ax[i].extent), cb = cost(p.k_ax[i].sb, p.k_ax[i].extent);
+    if (ca < a_k) {
+      a_k = ca;
+      kA = (int)i;
+    }
+    if (cb < b_k) {
+      b_k = cb;
+      kB = (int)i;
+    }
+  }
+  p.a_kmajor = a_k <= a_m;
+  p.b_kmajor = b_k <= b_n;
+  if (!p.a_kmajor) kA = -1;
+  if (!p.b_kmajor) kB = -1;
+  double d_m = p.m_ax.empty() ? inf : cost(p.m_ax[0].sd, p.m_ax[0].extent);
+  double d_n = p.n_ax.empty() ? inf : cost(p.n_ax[0].sd, p.n_ax[0].extent);
+  p.d_mmajor = d_m <= d_n;
+
+  // K order.
+  std::vector<Axis> korder;
+  if (kA >= 0 && kB >= 0 && kA != kB) {
+    const Axis &xa = p.k_ax[kA], &xb = p.k_ax[kB];
+    int64_t lineA = std::max<int64_t>(1, 128 / std::max<int64_t>(1, (int64_t)uabs(xa.sa) * es));
+    int64_t lineB = std::max<int64_t>(1, 128 / std::max<int64_t>(1, (int64_t)uabs(xb.sb) * es));
+    int64_t aLo = largest_divisor_le(xa.extent, std::min(BK, lineA));
+    int64_t bLo = largest_divisor_le(xb.extent, std::max<int64_t>(1, BK / aLo));
+    int64_t bMid = largest_divisor_le(xb.extent / bLo, std::max<int64_t>(1, lineB / bLo));
+    push_nonunit(korder, scaled(xa, aLo, 1));
+    push_nonunit(korder, scaled(xb, bLo, 1));
+    push_nonunit(korder, scaled(xb, bMid, bLo));
+    push_nonunit(korder, scaled(xa, xa.extent / aLo, aLo));
+    push_nonunit(korder, scaled(xb, xb.extent / (bLo * bMid), bLo * bMid));
+    for (size_t i = 0; i < p.k_ax.size(); i++)
+      if ((int)i != kA && (int)i != kB) korder.push_back(p.k_ax[i]);
+  } else {
+    int first = kA >= 0 ? kA : kB;
+    if (first >= 0) korder.push_back(p.k_ax[first]);
+    for (size_t i = 0; i < p.k_ax.size(); i++)
+      if ((int)i != first) korder.push_back(p.k_ax[i]);
+  }
+
+  p.am = scatter_for(p.m_ax, [](const Axis& a) { return a.sa; });
+  p.cm = scatter_for(p.m_ax, [](const Axis& a) { return a.sc; });
+  p.dm = scatter_for(p.m_ax, [](const Axis& a) { return a.sd; });
+  p.bn = scatter_for(p.n_ax, [](const Axis& a) { return a.sb; });
+  p.cn = scatter_for(p.n_ax, [](const Axis& a) { return a.sc; });
+  p.dn = scatter_for(p.n_ax, [](const Axis& a) { return a.sd; });
+  p.ak = scatter_for(korder, [](const Axis& a) { return a.sa; });
+  p.bk = scatter_for(korder, [](const Axis& a) { return a.sb; });
+  p.ha = scatter_for(p.h_ax, [](const Axis& a) { return a.sa; });
+  p.hb = scatter_for(p.h_ax, [](const Axis& a) { return a.sb; });
+  p.hc = scatter_for(p.h_ax, [](const Axis& a) { return a.sc; });
+  p.hd = scatter_for(p.h_ax, [](const Axis& a) { return a.sd; });
+  p.dr = scatter_for(p.r_ax, [](const Axis& a) { return a.sd; });
+  // Empty roles would give empty vectors; keep one zero entry so device
+  // pointers stay valid (the kernels never index past M, N, K, H).
+  for (auto* v : {&p.am, &p.cm, &p.dm, &p.bn, &p.cn, &p.dn, &p.ak, &p.bk, &p.ha, &p.hb, &p.hc, &p.hd, &p.dr})
+    if (v->empty()) v->push_back(0);
+}
+
+Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
+  std::unique_ptr<Plan> p(new Plan);
+  p->prob = pr;
+  const auto& r = pr.roles;
+  p->m_ax = role_axes(r.m, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sa)); });
+  p->n_ax = role_axes(r.n, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sb)); });
+  p->k_ax = role_axes(r.k, [](const Axis& x) { return std::make_pair(uabs(x.sa), uabs(x.sb)); });
+  p->h_ax = role_axes(r.h, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sa)); });
+  p->r_ax = role_axes(r.r, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sc)); });
+  p->M = product(p->m_ax);
+  p->N = product(p->n_ax);
+  p->K = product(p->k_ax);
+  p->H = product(p->h_ax);
+  p->R = product(p->r_ax);
+  if (pr.out_empty) p->strategy = 2;
+  else if (pr.k_empty) p->strategy = 1;
+  else p->strategy = 0;
+  if (p->strategy == 2) {
+    // Nothing is ever launched; keep the vectors trivial.
+    p->cfg = cfg;
+    p->tile = TILE_64x64;
+    for (auto* v : {&p->am, &p->cm, &p->dm, &p->bn, &p->cn, &p->dn, &p->ak, &p->bk, &p->ha, &p->hb, &p->hc, &p->hd,
+                    &p->dr})
+      v->assign(1, 0);
+    return p.release();
+  }
+  configure(p.get(), cfg);
+  return p.release();
+}
+
+const DeviceScatter& Plan::on_device(int device) {
+  std::lock_guard<std::mutex> g(mu);
+  auto it = dev.find(device);
+  if (it != dev.end()) return it->second;
+  std::vector<const std::vector<int64_t>*> parts = {&am, &ak, &bk, &bn, &cm, &cn, &dm, &dn, &ha, &hb, &hc, &hd, &dr};
+  size_t total = 0;
+  for (auto* v : parts) total += v->size();
+  std::vector<int64_t> flat;
+  flat.reserve(total);
+  for (auto* v : parts) flat.insert(flat.end(), v->begin(), v->end());
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != device) (void)hipSetDevice(device);
+  DeviceScatter ds{};
+  hipError_t e = hipMalloc(&ds.base, total * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMemcpy(ds.base, flat.data(), total * sizeof(int64_t), hipMemcpyHostToDevice);
+  if (prev != device) (void)hipSetDevice(prev);
+  if (e != hipSuccess) {
+    if (ds.base) (void)hipFree(ds.base);
+    fail(ERR_INTERNAL, std::string("uploading scatter vectors: ") + hipGetErrorString(e));
+  }
+  const int64_t* q = ds.base;
+  const int64_t** slots[] = {&ds.am, &ds.ak, &ds.bk, &ds.bn, &ds.cm, &ds.cn, &ds.dm,
+                             &ds.dn, &ds.ha, &ds.hb, &ds.hc, &ds.hd, &ds.dr};
+  for (size_t i = 0; i < parts.size(); i++) {
+    *slots[i] = q;
+    q += parts[i]->size();
+  }
+  return dev.emplace(device, ds).first->second;
+}
+
+ContractArgs Plan::args(int device, const void* A, const void* B, const void* C, void* D, const double alpha[2],
+                        const double beta[2], bool read_c) {
+  const DeviceScatter& s = on_device(device);
+  ContractArgs a{};
+  a.A = A;
+  a.B = B;
+  a.C = C;
+  a.D = D;
+  a.am = s.am;
+  a.ak = s.ak;
+  a.bk = s.bk;
+  a.bn = s.bn;
+  a.cm = s.cm;
+  a.cn = s.cn;
+  a.dm = s.dm;
+  a.dn = s.dn;
+  a.ha = s.ha;
+  a.hb = s.hb;
+  a.hc = s.hc;
+  a.hd = s.hd;
+  a.dr = r_ax.empty() ? nullptr : s.dr;
+  a.M = M;
+  a.N = N;
+  a.K = K;
+  a.H = H;
+  a.R = R;
+  a.alpha[0] = alpha[0];
+  a.alpha[1] = alpha[1];
+  a.beta[0] = beta[0];
+  a.beta[1] = beta[1];
+  a.conj_a = prob.a.conj && is_complex(prob.dtype);
+  a.conj_b = prob.b.conj && is_complex(prob.dtype);
+  a.conj_c = prob.c.conj && is_complex(prob.dtype);
+  a.conj_d = prob.d.conj && is_complex(prob.dtype);
+  a.a_kmajor = a_kmajor;
+  a.b_kmajor = b_kmajor;
+  a.d_mmajor = d_mmajor;
+  a.read_c = read_c;
+  return a;
+}
+
+std::string Plan::kernel_name() const {
+  if (strategy == 2) return "none";
+  if (strategy == 1) return std::string("output_pass.") + dtype_name(prob.dtype);
+  std::string s = dtype_name(prob.dtype);
+  if (cfg.precision == PREC_SPLIT_BF16_2) s += ".split_bf16x2";
+  else if (cfg.precision == PREC_SPLIT_BF16_3) s += ".split_bf16x3";
+  else s += is_double(prob.dtype) ? ".mfma_f64" : ".mfma_f32";
+  if (is_complex(prob.dtype)) {
+    static const char* m[3] = {".native", ".1m", ".3m"};
+    s += m[cfg.complex_method];
+  }
+  const TileCfg* t = tile_cfg(tile);
+  if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(t->bk);
+  return s;
+}
+
+Plan::~Plan() {
+  for (auto& [d, s] : dev) {
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (prev != d) (void)hipSetDevice(d);
+    (void)hipFree(s.base);
+    if (prev != d) (void)hipSetDevice(prev);
+  }
+}
+
+}  // namespace tp

@@ -1,0 +1,71 @@
+// The GPU plan of one contraction: folded role axes, scatter vectors, gather
+// orientation, K order and tile choice. Everything here depends only on shapes,
+// strides and labels (crates/tprims-contract/src/plan/analysis.rs:210 PackedPlan,
+// crates/tprims-contract/src/plan/orientation.rs, crates/tprims-contract/src/resolve.rs:42).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kargs.h"
+#include "problem.h"
+
+namespace tp {
+
+// One folded axis (analysis.rs:63-76): its extent and strides in A, B, C, D.
+struct Axis {
+  int64_t extent;
+  int64_t sa, sb, sc, sd;
+};
+
+struct PlanConfig {
+  int32_t complex_method = CPLX_NATIVE;
+  int32_t precision = PREC_STORAGE;
+  int32_t tile = 0;  // 0: chosen by the planner
+};
+
+// Scatter vectors of a plan resident on one device (one allocation).
+struct DeviceScatter {
+  int64_t* base = nullptr;
+  const int64_t *am, *ak, *bk, *bn, *cm, *cn, *dm, *dn, *ha, *hb, *hc, *hd, *dr;
+};
+
+struct Plan {
+  Problem prob;
+  PlanConfig cfg;
+  std::vector<Axis> m_ax, n_ax, k_ax, h_ax, r_ax;
+  int64_t M = 1, N = 1, K = 1, H = 1, R = 1;
+  std::vector<int64_t> am, ak, bk, bn, cm, cn, dm, dn, ha, hb, hc, hd, dr;
+  bool a_kmajor = true, b_kmajor = true, d_mmajor = true;
+  int tile = 0;
+  int strategy = 0;  // 0 contraction, 1 output pass only (K empty), 2 output empty
+
+  // Device copies, uploaded on first use per device.
+  std::mutex mu;
+  std::map<int, DeviceScatter> dev;
+
+  int64_t flops() const { return (is_complex(prob.dtype) ? 8 : 2) * M * N * K * H; }
+  const DeviceScatter& on_device(int device);
+  // Kernel arguments for data pointers (device addresses) and scalars.
+  ContractArgs args(int device, const void* A, const void* B, const void* C, void* D, const double alpha[2],
+                    const double beta[2], bool read_c);
+  std::string kernel_name() const;
+  ~Plan();
+};
+
+// Builds the plan of a validated problem.
+Plan* make_plan(const Problem& p, const PlanConfig& cfg);
+// Re-runs the tile choice and K ordering for a new configuration.
+void configure(Plan* plan, const PlanConfig& cfg);
+
+// The scatter vector of a mixed-radix index group, fastest axis first
+// (crates/tprims-kernel/src/pack/scatter.rs:70).
+std::vector<int64_t> build_scatter(const std::vector<int64_t>& extents, const std::vector<int64_t>& strides);
+// Folding of adjacent compatible axes (analysis.rs:299-317).
+std::vector<Axis> fold_axes(const std::vector<Axis>& axes);
+
+}  // namespace tp

@@ -1,0 +1,422 @@
+"""tprims_amd: the Python host mirror of the reference's contraction API,
+running on libtprims.so (hand-written HIP kernels for MI355X / gfx950).
+
+Mirrors crates/tprims-contract/src/api (Problem::from_labels,
+Problem::from_dot_general, Labels, DotGeneral, LayoutSpec, OperandSpec,
+CSpec) and crates/tprims-contract/src/plan (Plan::new, execute_into,
+execute_into_accum, execute_raw, report). A Plan lowers to one TAPP product of
+the C ABI; execution runs on the GPU through that product.
+
+Operands are torch tensors (device or host), numpy arrays (host, staged by the
+library) or raw integer addresses (`execute_raw`). The library is required:
+nothing here computes a contraction in Python.
+"""
+import ctypes
+import enum
+
+from . import _capi
+from ._capi import TprimsError, check, i64_array, lib
+
+__all__ = [
+    "DType", "Op", "LayoutSpec", "OperandSpec", "CSpec", "Labels", "DotGeneral", "Problem", "PlanConfig", "Plan",
+    "Executor", "TprimsError", "ComplexMethod", "Precision", "col_major", "row_major",
+]
+
+
+class DType(enum.IntEnum):
+    """Storage type (crates/tprims-contract/src/api/problem.rs:17)."""
+    F32 = _capi.TAPP_F32
+    F64 = _capi.TAPP_F64
+    C32 = _capi.TAPP_C32
+    C64 = _capi.TAPP_C64
+
+    @property
+    def size(self):
+        return {DType.F32: 4, DType.F64: 8, DType.C32: 8, DType.C64: 16}[self]
+
+    @property
+    def is_complex(self):
+        return self in (DType.C32, DType.C64)
+
+    @staticmethod
+    def of(x):
+        """DType of a torch tensor or numpy array."""
+        name = str(x.dtype).replace("torch.", "")
+        table = {"float32": DType.F32, "float64": DType.F64, "complex64": DType.C32, "complex128": DType.C64}
+        if name not in table:
+            raise TprimsError(_capi.TPRIMS_ERR_DTYPE, "unsupported dtype %s" % name)
+        return table[name]
+
+
+class Op(enum.IntEnum):
+    IDENTITY = _capi.TAPP_IDENTITY
+    CONJUGATE = _capi.TAPP_CONJUGATE
+
+
+class ComplexMethod(enum.IntEnum):
+    NATIVE = _capi.TPRIMS_CPLX_NATIVE
+    ONE_M = _capi.TPRIMS_CPLX_1M
+    THREE_M = _capi.TPRIMS_CPLX_3M
+
+
+class Precision(enum.IntEnum):
+    STORAGE = _capi.TPRIMS_PREC_STORAGE
+    SPLIT_BF16_2 = _capi.TPRIMS_PREC_SPLIT_BF16_2
+    SPLIT_BF16_3 = _capi.TPRIMS_PREC_SPLIT_BF16_3
+    FP8_E4M3 = _capi.TPRIMS_PREC_FP8_E4M3
+
+
+def col_major(dims):
+    """Column-major element strides (fixtures.rs:44)."""
+    s, acc = [], 1
+    for d in dims:
+        s.append(acc)
+        acc *= max(int(d), 1)
+    return s
+
+
+def row_major(dims):
+    """Row-major element strides (fixtures.rs:55)."""
+    s, acc = [0] * len(dims), 1
+    for k in range(len(dims) - 1, -1, -1):
+        s[k] = acc
+        acc *= max(int(dims[k]), 1)
+    return s
+
+
+class LayoutSpec:
+    """Extents, signed element strides and a logical offset (problem.rs:71)."""
+
+    def __init__(self, dims, strides, offset=0):
+        if len(dims) != len(strides):
+            raise TprimsError(_capi.TPRIMS_ERR_SHAPE, "extents and strides differ in length")
+        self.dims = [int(d) for d in dims]
+        self.strides = [int(s) for s in strides]
+        self.offset = int(offset)
+
+    @staticmethod
+    def of(t):
+        """The layout of a torch tensor or numpy array (origin at its first element)."""
+        if hasattr(t, "stride") and callable(t.stride):
+            return LayoutSpec(list(t.shape), list(t.stride()), 0)
+        item = t.itemsize
+        return LayoutSpec(list(t.shape), [s // item for s in t.strides], 0)
+
+    @property
+    def rank(self):
+        return len(self.dims)
+
+    def __repr__(self):
+        return "LayoutSpec(%r, %r, %r)" % (self.dims, self.strides, self.offset)
+
+
+class OperandSpec:
+    """A layout plus an element-wise operation (problem.rs:196)."""
+
+    def __init__(self, layout, op=Op.IDENTITY):
+        self.layout = layout
+        self.op = Op(op)
+
+    def conj(self):
+        return OperandSpec(self.layout, Op.CONJUGATE)
+
+
+class CSpec:
+    """How beta*op_C(C) is supplied (problem.rs:236): Absent, Output(op), Separate(spec)."""
+
+    def __init__(self, kind, op=Op.IDENTITY, spec=None):
+        self.kind, self.op, self.spec = kind, Op(op), spec
+
+    @staticmethod
+    def absent():
+        return CSpec("absent")
+
+    @staticmethod
+    def output(op=Op.IDENTITY):
+        return CSpec("output", op)
+
+    @staticmethod
+    def separate(spec):
+        return CSpec("separate", spec.op, spec)
+
+
+class Labels:
+    """Per-operand int64 labels (crates/tprims-contract/src/api/labels.rs:19)."""
+
+    def __init__(self, a, b, d, c=None):
+        self.a = [int(x) for x in a]
+        self.b = [int(x) for x in b]
+        self.d = [int(x) for x in d]
+        self.c = None if c is None else [int(x) for x in c]
+
+    def with_c(self, c):
+        return Labels(self.a, self.b, self.d, c)
+
+
+class DotGeneral:
+    """Contracted and batch axis lists (crates/tprims-contract/src/api/dot_general.rs:20).
+
+    Output extents are [lhs_free, rhs_free, batch]; C is D.
+    """
+
+    def __init__(self, lhs_contract, rhs_contract, lhs_batch=(), rhs_batch=()):
+        self.lhs_contract = [int(x) for x in lhs_contract]
+        self.rhs_contract = [int(x) for x in rhs_contract]
+        self.lhs_batch = [int(x) for x in lhs_batch]
+        self.rhs_batch = [int(x) for x in rhs_batch]
+
+    def validate(self, a_dims, b_dims):
+        """dot_general.rs:82-141: returns (lhs_free, rhs_free, out_dims)."""
+        for what, l, r in (("contracting", self.lhs_contract, self.rhs_contract),
+                           ("batch", self.lhs_batch, self.rhs_batch)):
+            if len(l) != len(r):
+                raise TprimsError(_capi.TPRIMS_ERR_INVALID_ARGUMENT, "%s axis lists differ in length" % what)
+
+        def free(name, rank, c, bt):
+            seen = [False] * rank
+            for x in list(c) + list(bt):
+                if x >= rank or x < 0:
+                    raise TprimsError(_capi.TPRIMS_ERR_INVALID_ARGUMENT, "axis %d out of range for %s" % (x, name))
+                if seen[x]:
+                    raise TprimsError(_capi.TPRIMS_ERR_INVALID_ARGUMENT, "axis %d of %s repeated" % (x, name))
+                seen[x] = True
+            return [x for x in range(rank) if not seen[x]]
+
+        lf = free("A", len(a_dims), self.lhs_contract, self.lhs_batch)
+        rf = free("B", len(b_dims), self.rhs_contract, self.rhs_batch)
+        for l, r in list(zip(self.lhs_contract, self.rhs_contract)) + list(zip(self.lhs_batch, self.rhs_batch)):
+            if a_dims[l] != b_dims[r]:
+                raise TprimsError(_capi.TPRIMS_ERR_SHAPE, "paired axes %d/%d have extents %d and %d"
+                                  % (l, r, a_dims[l], b_dims[r]))
+        out = [a_dims[x] for x in lf] + [b_dims[x] for x in rf] + [a_dims[x] for x in self.lhs_batch]
+        return lf, rf, out
+
+
+class Problem:
+    """D = op_D(alpha * sum_K op_A(A) op_B(B) + beta * op_C(C)) (problem.rs:1-10).
+
+    Validation happens when the Plan lowers it through the C ABI.
+    """
+
+    def __init__(self, dtype, a, b, c, d, labels):
+        self.dtype = DType(dtype)
+        self.a, self.b, self.c, self.d, self.labels = a, b, c, d, labels
+        if (c.kind == "separate") != (labels.c is not None):
+            raise TprimsError(_capi.TPRIMS_ERR_LABELS, "a separate C needs its labels")
+
+    @staticmethod
+    def from_labels(dtype, a, b, c, d, labels):
+        return Problem(dtype, a, b, c, d, labels)
+
+    @staticmethod
+    def from_dot_general(dtype, a, b, d, dot):
+        """problem.rs:396-449: synthetic labels, C is D with identity op_C."""
+        lf, rf, out = dot.validate(a.layout.dims, b.layout.dims)
+        if out != d.layout.dims:
+            raise TprimsError(_capi.TPRIMS_ERR_SHAPE, "output extents %r, expected %r" % (d.layout.dims, out))
+        nb, nc = len(dot.lhs_batch), len(dot.lhs_contract)
+        la, lb, ld = [0] * a.layout.rank, [0] * b.layout.rank, []
+        nxt = nb + nc
+        for x in lf:
+            la[x] = nxt
+            ld.append(nxt)
+            nxt += 1
+        for x in rf:
+            lb[x] = nxt
+            ld.append(nxt)
+            nxt += 1
+        for p, (l, r) in enumerate(zip(dot.lhs_batch, dot.rhs_batch)):
+            la[l] = p
+            lb[r] = p
+            ld.append(p)
+        for p, (l, r) in enumerate(zip(dot.lhs_contract, dot.rhs_contract)):
+            la[l] = nb + p
+            lb[r] = nb + p
+        return Problem(dtype, a, b, CSpec.output(Op.IDENTITY), d, Labels(la, lb, ld))
+
+
+class PlanConfig:
+    """Plan configuration (crates/tprims-contract/src/plan/config.rs:101), GPU knobs."""
+
+    def __init__(self, complex_method=ComplexMethod.NATIVE, precision=Precision.STORAGE, tile=0):
+        self.complex_method = int(complex_method)
+        self.precision = int(precision)
+        self.tile = int(tile)
+
+    def to_c(self):
+        c = _capi.PlanConfigC()
+        c.complex_method, c.precision, c.tile = self.complex_method, self.precision, self.tile
+        return c
+
+
+class Executor:
+    """A GPU executor: a device and a stream (tprims_tapp_executor_create_hip)."""
+
+    def __init__(self, device=0, stream=None, asynchronous=False):
+        h = _capi.intptr(0)
+        check(lib().tprims_tapp_executor_create_hip(ctypes.byref(h), int(device), stream))
+        self.handle = h.value
+        if asynchronous:
+            check(lib().tprims_tapp_executor_set_async(self.handle, 1))
+
+    def synchronize(self):
+        check(lib().tprims_tapp_executor_synchronize(self.handle))
+
+    @property
+    def stream(self):
+        s = ctypes.c_void_p(0)
+        check(lib().tprims_tapp_executor_get_stream(self.handle, ctypes.byref(s)))
+        return s.value
+
+    def close(self):
+        if self.handle:
+            check(lib().TAPP_destroy_executor(self.handle))
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_HANDLE = None
+
+
+def _handle():
+    global _HANDLE
+    if _HANDLE is None:
+        h = _capi.intptr(0)
+        check(lib().TAPP_create_handle(ctypes.byref(h)))
+        _HANDLE = h.value
+    return _HANDLE
+
+
+def _info(dtype, layout):
+    h = _capi.intptr(0)
+    check(lib().TAPP_create_tensor_info(ctypes.byref(h), int(dtype), layout.rank, i64_array(layout.dims),
+                                        i64_array(layout.strides)))
+    return h.value
+
+
+def _addr(x, layout=None):
+    """Address of the element at logical index zero of a tensor/array/int."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    return x.ctypes.data  # numpy
+
+
+def _scalar(v, dtype):
+    v = complex(v)
+    if dtype == DType.F32:
+        return ctypes.byref(ctypes.c_float(v.real)), ctypes.c_float(v.real)
+    if dtype == DType.F64:
+        c = ctypes.c_double(v.real)
+        return ctypes.byref(c), c
+    if dtype == DType.C32:
+        c = (ctypes.c_float * 2)(v.real, v.imag)
+        return ctypes.cast(c, ctypes.c_void_p), c
+    c = (ctypes.c_double * 2)(v.real, v.imag)
+    return ctypes.cast(c, ctypes.c_void_p), c
+
+
+class Plan:
+    """A prepared contraction (crates/tprims-contract/src/plan/mod.rs:139).
+
+    Holds no data pointer; reusable across buffers and executors.
+    """
+
+    def __init__(self, problem, config=None):
+        self.problem = problem
+        self.config = config or PlanConfig()
+        p = problem
+        la, lb, ld = p.labels.a, p.labels.b, p.labels.d
+        if p.c.kind == "separate":
+            c_layout, lc, c_op = p.c.spec.layout, p.labels.c, p.c.op
+        else:
+            c_layout, lc, c_op = p.d.layout, ld, p.c.op
+        self._c_kind = p.c.kind
+        infos = [_info(p.dtype, l) for l in (p.a.layout, p.b.layout, c_layout, p.d.layout)]
+        try:
+            h = _capi.intptr(0)
+            cfg = self.config.to_c()
+            check(lib().tprims_create_tensor_product_ex(
+                ctypes.byref(h), _handle(), int(p.a.op), infos[0], i64_array(la), int(p.b.op), infos[1],
+                i64_array(lb), int(c_op), infos[2], i64_array(lc), int(p.d.op), infos[3], i64_array(ld),
+                _capi.TAPP_DEFAULT_PREC, ctypes.byref(cfg)))
+            self.handle = h.value
+        finally:
+            for i in infos:
+                lib().TAPP_destroy_tensor_info(i)
+
+    @staticmethod
+    def new(problem, config=None):
+        return Plan(problem, config)
+
+    def report(self):
+        """The planner's conclusions (plan/report.rs:33) as a dict."""
+        r = _capi.PlanReportC()
+        check(lib().tprims_product_report(self.handle, ctypes.byref(r)))
+        out = {f: getattr(r, f) for f, _ in _capi.PlanReportC._fields_}
+        out["kernel"] = r.kernel.decode()
+        return out
+
+    def execute_raw(self, alpha, a, b, beta, c, d, executor=None):
+        """Execute on element addresses of index zero (plan/mod.rs:452)."""
+        dt = self.problem.dtype
+        pa, _ka = _scalar(alpha, dt)
+        pb, _kb = _scalar(beta, dt)
+        st = _capi.intptr(0)
+        ex = executor.handle if executor is not None else 0
+        check(lib().TAPP_execute_product(self.handle, ex, ctypes.byref(st), pa, a, b, pb, c, d))
+
+    def execute_into(self, alpha, a, b, d, executor=None):
+        """D = op_D(alpha * op_A(A) op_B(B)) (plan/mod.rs:271); C is not read."""
+        self.execute_raw(alpha, _addr(a), _addr(b), 0.0, None, _addr(d), executor)
+
+    def execute_into_accum(self, alpha, a, b, beta, c, d, executor=None):
+        """D = op_D(alpha * op_A(A) op_B(B) + beta * op_C(C)) (plan/mod.rs:309).
+
+        For a problem whose C is D (CSpec.output), pass c=None: D is read in place.
+        """
+        if self._c_kind == "absent":
+            beta, c = 0.0, None
+        elif self._c_kind == "output":
+            c = d
+        self.execute_raw(alpha, _addr(a), _addr(b), beta, _addr(c), _addr(d), executor)
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            lib().TAPP_destroy_tensor_product(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def contract(labels_a, a, labels_b, b, labels_d, d, alpha=1.0, beta=0.0, c=None, labels_c=None, conj_a=False,
+             conj_b=False, conj_c=False, conj_d=False, config=None, executor=None):
+    """One-shot label contraction on torch tensors / numpy arrays."""
+    dt = DType.of(d)
+    spec = lambda x, cj: OperandSpec(LayoutSpec.of(x), Op.CONJUGATE if cj else Op.IDENTITY)
+    if c is not None:
+        cs = CSpec.separate(spec(c, conj_c))
+        lab = Labels(labels_a, labels_b, labels_d, labels_c if labels_c is not None else labels_d)
+    else:
+        cs = CSpec.absent()
+        lab = Labels(labels_a, labels_b, labels_d)
+    if c is None:
+        # TAPP needs a C description: reuse D's, unread at beta 0.
+        cs = CSpec.separate(spec(d, conj_c))
+        lab = lab.with_c(labels_d)
+        beta = 0.0
+    plan = Plan(Problem.from_labels(dt, spec(a, conj_a), spec(b, conj_b), cs, spec(d, conj_d), lab), config)
+    plan.execute_raw(alpha, _addr(a), _addr(b), beta, _addr(c), _addr(d), executor)
+    return plan

@@ -1,0 +1,296 @@
+"""CPU tests of the C ABI: the library loads and exports every symbol the
+headers declare; handles, tensor infos, executors and products behave as the
+reference's (crates/tprims-capi/tests/tapp_rejections.rs, tapp_conformance.rs,
+c/abi_test.c); every validation error fires before any GPU work; and the
+planner's index analysis matches the reference's analysis goldens
+(crates/tprims-contract/src/plan/analysis.rs:323-388)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import tprims_amd as tp
+from tprims_amd import _capi
+from tprims_amd._capi import TprimsError, i64_array, intptr, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    names = set()
+    for root, _, files in os.walk(os.path.join(ROOT, "include")):
+        for f in files:
+            if f.endswith(".h"):
+                text = open(os.path.join(root, f)).read()
+                names.update(re.findall(r"TAPP_EXPORT\s+[\w\s\*]+?\b(\w+)\s*\(", text))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    out = subprocess.check_output(["nm", "-D", "--defined-only", _capi.LIB_PATH]).decode()
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    declared = header_symbols()
+    assert len(declared) >= 45
+    missing = declared - exported
+    assert not missing, missing
+    assert set(_capi.EXPORTED) <= exported
+
+
+def test_identity_and_parts():
+    l = lib()
+    assert l.tprims_abi_version() == 100
+    assert l.tprims_has_part(b"tapp") and l.tprims_has_part(b"core") and l.tprims_has_part(b"hip")
+    assert not l.tprims_has_part(b"blas") and not l.tprims_has_part(None)
+    assert l.TAPP_implementation_name() and l.TAPP_implementation_version() == b"0.1.0"
+    buf = ctypes.create_string_buffer(64)
+    n = l.TAPP_explain_error(_capi.TPRIMS_BUSY, 64, buf)
+    assert n == len(buf.value) > 0
+    assert l.TAPP_explain_error(_capi.TPRIMS_BUSY, 0, None) == n
+    small = ctypes.create_string_buffer(4)
+    assert l.TAPP_explain_error(_capi.TPRIMS_ERR_ALIASED, 4, small) > 3 and len(small.value) == 3
+    assert l.TAPP_check_success(0) and not l.TAPP_check_success(1)
+
+
+def test_handles_attrs_status():
+    l = lib()
+    h = intptr(0)
+    assert l.TAPP_create_handle(ctypes.byref(h)) == 0 and h.value != 0
+    h2 = intptr(0)
+    assert l.TAPP_create_handle(ctypes.byref(h2)) == 0 and h2.value != h.value
+    assert l.TAPP_create_handle(None) == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    v = ctypes.c_void_p(123)
+    assert l.TAPP_attr_get(h.value, 0, ctypes.byref(v)) == _capi.TPRIMS_ERR_UNSUPPORTED and v.value is None
+    assert l.TAPP_attr_set(h.value, 0, None) == _capi.TPRIMS_ERR_UNSUPPORTED
+    assert l.TAPP_attr_clear(h.value, 0) == _capi.TPRIMS_ERR_UNSUPPORTED
+    assert l.tprims_last_error() == b"no attribute keys are defined"
+    assert l.TAPP_destroy_status(12345) == 0 and l.tprims_last_error() == b""
+    assert l.TAPP_destroy_handle(h.value) == 0 and l.TAPP_destroy_handle(h2.value) == 0
+    assert l.TAPP_destroy_handle(0) == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+
+
+def test_tensor_info_lifecycle():
+    l = lib()
+    info = intptr(0)
+    assert l.TAPP_create_tensor_info(ctypes.byref(info), _capi.TAPP_C64, 3, i64_array([2, 3, 4]),
+                                     i64_array([1, 2, 6])) == 0
+    assert l.TAPP_get_nmodes(info.value) == 3
+    e = (ctypes.c_int64 * 3)()
+    l.TAPP_get_extents(info.value, e)
+    assert list(e) == [2, 3, 4]
+    assert l.TAPP_set_strides(info.value, i64_array([3, 1, 9])) == 0
+    s = (ctypes.c_int64 * 3)()
+    l.TAPP_get_strides(info.value, s)
+    assert list(s) == [3, 1, 9]
+    assert l.TAPP_set_nmodes(info.value, 4) == 0
+    e4, s4 = (ctypes.c_int64 * 4)(), (ctypes.c_int64 * 4)()
+    l.TAPP_get_extents(info.value, e4)
+    l.TAPP_get_strides(info.value, s4)
+    assert list(e4) == [2, 3, 4, 1] and list(s4) == [3, 1, 9, 0]
+    assert l.TAPP_set_nmodes(info.value, -1) == _capi.TPRIMS_ERR_SHAPE
+    assert l.TAPP_set_extents(info.value, None) == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    assert l.TAPP_get_nmodes(0) == -1
+    assert l.TAPP_destroy_tensor_info(info.value) == 0
+    bad = intptr(0)
+    assert l.TAPP_create_tensor_info(ctypes.byref(bad), _capi.TAPP_F16, 1, i64_array([2]),
+                                     i64_array([1])) == _capi.TPRIMS_ERR_DTYPE
+    assert l.TAPP_create_tensor_info(ctypes.byref(bad), _capi.TAPP_BF16, 0, None, None) == _capi.TPRIMS_ERR_DTYPE
+    assert l.TAPP_create_tensor_info(ctypes.byref(bad), _capi.TAPP_F32, -1, None, None) == _capi.TPRIMS_ERR_SHAPE
+    assert l.TAPP_create_tensor_info(ctypes.byref(bad), _capi.TAPP_F32, 2, None, None) == \
+        _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    assert l.TAPP_create_tensor_info(ctypes.byref(bad), _capi.TAPP_F32, 0, None, None) == 0
+
+
+def test_executor_thread_settings():
+    # crates/tprims-capi/tests/c/abi_test.c executor section
+    l = lib()
+    pool, serial = intptr(0), intptr(0)
+    assert l.tprims_tapp_executor_create_rayon(ctypes.byref(pool), 4, None) == 0 and pool.value
+    ps, bd = ctypes.c_size_t(99), ctypes.c_size_t(99)
+    assert l.tprims_tapp_executor_get_threads(pool.value, ctypes.byref(ps), ctypes.byref(bd)) == 0
+    assert (ps.value, bd.value) == (4, 4)
+    assert l.tprims_tapp_executor_set_budget(pool.value, 2) == 0
+    l.tprims_tapp_executor_get_threads(pool.value, ctypes.byref(ps), ctypes.byref(bd))
+    assert (ps.value, bd.value) == (4, 2)
+    assert l.tprims_tapp_executor_set_budget(pool.value, 0) == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    assert l.tprims_tapp_executor_set_budget(pool.value, 99) == 0
+    l.tprims_tapp_executor_get_threads(pool.value, ctypes.byref(ps), ctypes.byref(bd))
+    assert (ps.value, bd.value) == (4, 4)
+    assert l.TAPP_create_executor(ctypes.byref(serial)) == 0
+    l.tprims_tapp_executor_get_threads(serial.value, ctypes.byref(ps), ctypes.byref(bd))
+    assert (ps.value, bd.value) == (0, 1)
+    l.tprims_tapp_executor_get_threads(0, ctypes.byref(ps), ctypes.byref(bd))
+    assert (ps.value, bd.value) == (0, 1)
+    assert l.tprims_tapp_executor_create_rayon(ctypes.byref(pool), 0, None) == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    assert l.TAPP_destroy_executor(pool.value) == 0
+    assert l.TAPP_destroy_executor(serial.value) == 0
+    assert l.TAPP_destroy_executor(0) == 0
+
+
+def _plan(dtype, a, la, b, lb, d, ld, c=None, lc=None, conj=(0, 0, 0, 0), prec=_capi.TAPP_DEFAULT_PREC,
+          dtypes=None):
+    l = lib()
+    h = intptr(0)
+    l.TAPP_create_handle(ctypes.byref(h))
+    infos = []
+    dts = dtypes or [dtype] * 4
+    for (ext, st), dt in zip([a, b, c or d, d], dts):
+        i = intptr(0)
+        assert l.TAPP_create_tensor_info(ctypes.byref(i), dt, len(ext), i64_array(ext), i64_array(st)) == 0
+        infos.append(i.value)
+    p = intptr(0)
+    status = l.TAPP_create_tensor_product(ctypes.byref(p), h.value, conj[0], infos[0], i64_array(la), conj[1],
+                                          infos[1], i64_array(lb), conj[2], infos[2], i64_array(lc or ld), conj[3],
+                                          infos[3], i64_array(ld), prec)
+    for i in infos:
+        l.TAPP_destroy_tensor_info(i)
+    return status, p.value
+
+
+M44 = ([4, 4], [1, 4])
+
+
+def test_product_rejections():
+    F64, F32 = _capi.TAPP_F64, _capi.TAPP_F32
+    g = ([2, 3], [1, 2]), [0, 2], ([3, 2], [1, 3]), [2, 1], ([2, 2], [1, 2]), [0, 1]
+    st, _ = _plan(F64, *g, dtypes=[F64, F32, F64, F64])
+    assert st == _capi.TPRIMS_ERR_DTYPE
+    st, _ = _plan(F64, *g, prec=_capi.TAPP_F32F32_ACCUM_F32)
+    assert st == _capi.TPRIMS_ERR_UNSUPPORTED
+    st, _ = _plan(F64, *g, prec=_capi.TAPP_BF16BF16_ACCUM_F32)
+    assert st == _capi.TPRIMS_ERR_UNSUPPORTED
+    st, _ = _plan(F64, *g, conj=(2, 0, 0, 0))
+    assert st == _capi.TPRIMS_ERR_UNSUPPORTED
+    # an output-only label (TAPP case 5) is declined
+    st, _ = _plan(F64, ([2], [1]), [0], ([2], [1]), [0], ([2, 3], [1, 2]), [0, 9])
+    assert st == _capi.TPRIMS_ERR_UNSUPPORTED
+    # a label with two extents
+    st, _ = _plan(F64, ([2, 3], [1, 2]), [0, 2], ([4, 2], [1, 4]), [2, 1], ([2, 2], [1, 2]), [0, 1])
+    assert st == _capi.TPRIMS_ERR_SHAPE
+    # C and D labelled differently
+    st, _ = _plan(F64, *g, c=([2, 2], [1, 2]), lc=[0, 5])
+    assert st == _capi.TPRIMS_ERR_LABELS
+    # an output that overlaps itself
+    st, _ = _plan(F64, ([2, 3], [1, 2]), [0, 2], ([3, 2], [1, 3]), [2, 1], ([2, 2], [1, 1]), [0, 1])
+    assert st == _capi.TPRIMS_ERR_ALIASED
+    # negative extent
+    st, _ = _plan(F64, ([2, -3], [1, 2]), [0, 2], ([3, 2], [1, 3]), [2, 1], ([2, 2], [1, 2]), [0, 1])
+    assert st == _capi.TPRIMS_ERR_SHAPE
+    st, p = _plan(F64, *g)
+    assert st == 0
+    assert lib().TAPP_destroy_tensor_product(p) == 0
+    assert lib().TAPP_destroy_tensor_product(0) == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    # zero handle
+    l = lib()
+    i = intptr(0)
+    l.TAPP_create_tensor_info(ctypes.byref(i), F64, 2, i64_array([2, 2]), i64_array([1, 2]))
+    p = intptr(0)
+    assert l.TAPP_create_tensor_product(ctypes.byref(p), 0, 0, i.value, i64_array([0, 2]), 0, i.value,
+                                        i64_array([2, 1]), 0, i.value, i64_array([0, 1]), 0, i.value,
+                                        i64_array([0, 1]), -1) == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    assert l.TAPP_create_tensor_product(ctypes.byref(p), 1, 0, i.value, None, 0, i.value, i64_array([2, 1]), 0,
+                                        i.value, i64_array([0, 1]), 0, i.value, i64_array([0, 1]),
+                                        -1) == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    l.TAPP_destroy_tensor_info(i.value)
+
+
+def test_execute_validation_fires_before_any_device_work():
+    """Null pointers, a null C with beta != 0, and output aliasing are refused
+    from the pointers alone (execute.rs:30-47, plan/mod.rs:384-428)."""
+    F64 = _capi.TAPP_F64
+    l = lib()
+    st, p = _plan(F64, ([2, 3], [3, 1]), [0, 2], ([3, 2], [1, 3]), [2, 1], ([2, 2], [1, 2]), [0, 1])
+    assert st == 0
+    one, zero = ctypes.c_double(1.0), ctypes.c_double(0.0)
+    a = (ctypes.c_double * 6)(1, 3, 5, 2, 4, 6)
+    b = (ctypes.c_double * 6)(1, 3, 5, 2, 4, 6)
+    d = (ctypes.c_double * 4)()
+    status = intptr(-1)
+    r = l.TAPP_execute_product(p, 0, ctypes.byref(status), ctypes.byref(one), a, b, ctypes.byref(zero), None, a)
+    assert r == _capi.TPRIMS_ERR_ALIASED and status.value == 0
+    assert l.tprims_last_error()
+    r = l.TAPP_execute_product(p, 0, None, ctypes.byref(one), None, b, ctypes.byref(zero), None, d)
+    assert r == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    r = l.TAPP_execute_product(p, 0, None, ctypes.byref(one), a, b, ctypes.byref(one), None, d)
+    assert r == _capi.TPRIMS_ERR_UNSUPPORTED
+    r = l.TAPP_execute_product(p, 0, None, None, a, b, ctypes.byref(zero), None, d)
+    assert r == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    # a separate C that partially overlaps D
+    big = (ctypes.c_double * 8)()
+    c_ptr = ctypes.addressof(big) + 8
+    r = l.TAPP_execute_product(p, 0, None, ctypes.byref(one), a, b, ctypes.byref(one), c_ptr, big)
+    assert r == _capi.TPRIMS_ERR_ALIASED
+    # batched: a negative count, null arrays
+    assert l.TAPP_execute_batched_product(p, 0, None, -1, ctypes.byref(one), None, None, ctypes.byref(zero), None,
+                                          None) == _capi.TPRIMS_ERR_SHAPE
+    assert l.TAPP_execute_batched_product(p, 0, None, 1, ctypes.byref(one), None, None, ctypes.byref(zero), None,
+                                          None) == _capi.TPRIMS_ERR_INVALID_ARGUMENT
+    arr = (ctypes.c_void_p * 1)
+    assert l.TAPP_execute_batched_product(p, 0, None, 0, ctypes.byref(one), arr(), arr(), ctypes.byref(zero), None,
+                                          arr()) == 0
+    l.TAPP_destroy_tensor_product(p)
+
+
+def _report(dims_a, la, dims_b, lb, dims_d, ld, strides=None):
+    s = strides or (tp.col_major(dims_a), tp.col_major(dims_b), tp.col_major(dims_d))
+    A = tp.OperandSpec(tp.LayoutSpec(dims_a, s[0]))
+    B = tp.OperandSpec(tp.LayoutSpec(dims_b, s[1]))
+    D = tp.OperandSpec(tp.LayoutSpec(dims_d, s[2]))
+    prob = tp.Problem.from_labels(tp.DType.F64, A, B, tp.CSpec.separate(D), D, tp.Labels(la, lb, ld, ld))
+    return tp.Plan(prob).report()
+
+
+def test_analysis_goldens():
+    # analysis.rs:327-338 plain matmul folds to a pure GEMM (4, 6, 5, 1)
+    r = _report([4, 5], [0, 2], [5, 6], [2, 1], [4, 6], [0, 1])
+    assert (r["m"], r["n"], r["k"], r["batch"]) == (4, 6, 5, 1)
+    assert (r["m_axes"], r["n_axes"], r["k_axes"], r["h_axes"]) == (1, 1, 1, 0)
+    # analysis.rs:341-352 adjacent free indices fold
+    r = _report([4, 3, 5], [0, 1, 3], [5, 6], [3, 2], [4, 3, 6], [0, 1, 2])
+    assert r["m_axes"] == 1 and r["m"] == 12
+    # analysis.rs:355-364 Hadamard index recognised
+    r = _report([2, 4, 5], [9, 0, 3], [2, 5, 6], [9, 3, 1], [2, 4, 6], [9, 0, 1])
+    assert (r["m"], r["n"], r["k"], r["batch"]) == (4, 6, 5, 2)
+    # analysis.rs:367-377 isolated index becomes a zero-stride contraction: k = 15
+    r = _report([4, 5, 3], [0, 2, 7], [5, 6], [2, 1], [4, 6], [0, 1])
+    assert r["k"] == 15
+    # analysis.rs:380-387 repeated label takes the diagonal: k = 4
+    r = _report([4, 4], [0, 0], [4, 6], [0, 1], [6], [1])
+    assert (r["k"], r["m"], r["n"]) == (4, 1, 6)
+
+
+def test_benchmark_shapes_reduce_as_the_reference():
+    """The reduced roles of the benchmark contractions (folding included)."""
+    e = 24
+    r = _report([e] * 4, [0, 4, 1, 5], [e] * 4, [5, 2, 4, 3], [e] * 4, [0, 1, 2, 3])
+    assert (r["m"], r["n"], r["k"], r["batch"]) == (576, 576, 576, 1)
+    assert r["m_axes"] == 2 and r["n_axes"] == 2 and r["k_axes"] == 2
+    # complex benchmark: A (j,i,l,n) as every 2nd element of a 2x-padded buffer
+    j, i, l, n, k = 64, 512, 64, 16, 512
+    sa = [2, 2 * j, 2 * j * i, 2 * j * i * l]
+    A = tp.OperandSpec(tp.LayoutSpec([j, i, l, n], sa), tp.Op.CONJUGATE)
+    B = tp.OperandSpec(tp.LayoutSpec([l, k, j, n], tp.col_major([l, k, j, n])))
+    D = tp.OperandSpec(tp.LayoutSpec([i, k, n], tp.col_major([i, k, n])))
+    J, I, L, N, KK = 1, 2, 3, 4, 5
+    prob = tp.Problem.from_labels(tp.DType.C64, A, B, tp.CSpec.separate(D), D,
+                                  tp.Labels([J, I, L, N], [L, KK, J, N], [I, KK, N], [I, KK, N]))
+    r = tp.Plan(prob).report()
+    assert (r["m"], r["n"], r["k"], r["batch"]) == (512, 512, 4096, 16)
+    assert r["a_k_major"] == 1 and r["b_k_major"] == 1 and r["d_m_major"] == 1
+    assert r["flops"] == 8 * 512 * 512 * 4096 * 16
+
+
+def test_dot_general_mirror():
+    # dot_general.rs:14-18: bmm out dims [5, 2, 3]
+    dg = tp.DotGeneral([2], [1], [0], [0])
+    assert dg.validate([3, 5, 4], [3, 4, 2])[2] == [5, 2, 3]
+    with pytest.raises(TprimsError):
+        tp.DotGeneral([2], [1], [0], [0]).validate([3, 5, 4], [3, 5, 2])
+    A = tp.OperandSpec(tp.LayoutSpec([3, 5, 4], tp.col_major([3, 5, 4])))
+    B = tp.OperandSpec(tp.LayoutSpec([3, 4, 2], tp.col_major([3, 4, 2])))
+    D = tp.OperandSpec(tp.LayoutSpec([5, 2, 3], tp.col_major([5, 2, 3])))
+    p = tp.Plan(tp.Problem.from_dot_general(tp.DType.F64, A, B, D, dg))
+    r = p.report()
+    assert (r["m"], r["n"], r["k"], r["batch"]) == (5, 2, 4, 3)

@@ -1,0 +1,91 @@
+"""GPU parity: every conformance case, every dtype and complex method, through
+the C ABI (TAPP_execute_product) on the HIP kernels, against the FP64 label
+oracle over the whole output buffer."""
+import numpy as np
+import pytest
+
+import tprims_amd as tp
+from cases import CASES, scalar
+from harness import DTYPES, NP, T, check, run, values
+
+METHODS = [tp.ComplexMethod.NATIVE, tp.ComplexMethod.ONE_M, tp.ComplexMethod.THREE_M]
+
+
+def _variants():
+    out = []
+    for dt in DTYPES:
+        ms = METHODS if dt.is_complex else [tp.ComplexMethod.NATIVE]
+        for m in ms:
+            out.append((dt, m))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,method", _variants(), ids=lambda v: str(getattr(v, "name", v)))
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_conformance_case(gpu, name, dtype, method):
+    kw = CASES[name](dtype)
+    got, want, plan = run(dtype, config=tp.PlanConfig(complex_method=method), **kw)
+    exact = name in ("alpha_zero", "empty_k") and method != tp.ComplexMethod.THREE_M
+    check(dtype, got, want, method, exact=exact)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
+def test_single_complex_product_is_hand_checkable(gpu, dtype):
+    # tapp_conformance.rs:88-111: (1+2i)(3+4i) = -5+10i; 3 for real types.
+    a = T([1, 1], [1, 1], [0, 2])
+    b = T([1, 1], [1, 1], [2, 1])
+    d = T([1, 1], [1, 1], [0, 1])
+    av = np.array([scalar(dtype, 1, 2)], dtype=NP[dtype])
+    bv = np.array([scalar(dtype, 3, 4)], dtype=NP[dtype])
+    got, _, _ = run(dtype, a, av, b, bv, d, np.zeros(1, NP[dtype]))
+    want = complex(-5, 10) if dtype.is_complex else 3.0
+    assert got[0] == want
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
+def test_small_real_product_is_hand_checkable(gpu, dtype):
+    # tapp_conformance.rs:114-140: A B = [[19,22],[43,50]] column-major.
+    a = T([2, 2], [1, 2], [0, 2])
+    b = T([2, 2], [1, 2], [2, 1])
+    d = T([2, 2], [1, 2], [0, 1])
+    av = np.array([1, 3, 2, 4], dtype=NP[dtype])
+    bv = np.array([5, 7, 6, 8], dtype=NP[dtype])
+    got, _, _ = run(dtype, a, av, b, bv, d, np.zeros(4, NP[dtype]))
+    np.testing.assert_array_equal(got, np.array([19, 43, 22, 50], dtype=NP[dtype]))
+
+
+@pytest.mark.gpu
+def test_readme_c_example_values(gpu):
+    # crates/tprims-capi/tests/c/standard_consumer.c: D = [22, 28, 49, 64]
+    a = T([2, 3], [1, 2], [ord("i"), ord("k")])
+    b = T([3, 2], [1, 3], [ord("k"), ord("j")])
+    d = T([2, 2], [1, 2], [ord("i"), ord("j")])
+    av = np.arange(1, 7, dtype=np.float64)
+    got, _, _ = run(tp.DType.F64, a, av, b, av.copy(), d, np.full(4, -1.0))
+    np.testing.assert_array_equal(got, [22, 28, 49, 64])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
+@pytest.mark.parametrize("name", ["general_strides", "padded_view", "output_diagonal", "negative_strides",
+                                  "accumulate_in_place", "alpha_zero"])
+def test_host_operands_are_staged(gpu, name, dtype):
+    """Host (pageable) buffers go through the executor's device scratch."""
+    kw = CASES[name](dtype)
+    got, want, _ = run(dtype, where="host", **kw)
+    check(dtype, got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile", [1, 2, 3, 4])
+@pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
+def test_every_tile(gpu, tile, dtype):
+    if tile == 3 and dtype == tp.DType.C64:
+        pytest.skip("tile 3 is not offered for c64")
+    kw = CASES["large_k"](dtype)
+    got, want, plan = run(dtype, config=tp.PlanConfig(tile=tile), **kw)
+    assert plan.report()["tile"] == tile
+    check(dtype, got, want)
