@@ -30,12 +30,17 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mma;
+// Output placement of one 16x16 accumulator block: lane (lk = lane / 16,
+// column lane % 16) holds four rows. The f64 instruction interleaves them
+// (row lk + 4q); the f32 one keeps them together (row 4 lk + q). Both were
+// measured on gfx950 with known inputs.
 template <>
 struct Mma<double> {
   typedef v4d acc;
   __device__ static inline acc run(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int q) { return lk + 4 * q; }
 };
 template <>
 struct Mma<float> {
@@ -43,6 +48,7 @@ struct Mma<float> {
   __device__ static inline acc run(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int q) { return 4 * lk + q; }
 };
 
 template <typename T>
@@ -338,12 +344,14 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int col = wn + 16 * j + lr;
+        const int rr = MM::row(lk, q);  // row inside the 16x16 block
         if constexpr (C::METH == M_1M) {
-          const int row = wm + 8 * i + 2 * lk + (q >> 1);
-          if (q & 1) Dim[row * C::LDD + col] = acc[0][i][j][q];
+          // real row rr of the 1e product is complex row rr / 2, component rr % 2
+          const int row = wm + 8 * i + (rr >> 1);
+          if (rr & 1) Dim[row * C::LDD + col] = acc[0][i][j][q];
           else Dre[row * C::LDD + col] = acc[0][i][j][q];
         } else {
-          const int row = wm + 16 * i + 4 * lk + q;
+          const int row = wm + 16 * i + rr;
           if constexpr (C::METH == M_REAL) {
             Dre[row * C::LDD + col] = acc[0][i][j][q];
           } else if constexpr (C::METH == M_NATIVE) {
