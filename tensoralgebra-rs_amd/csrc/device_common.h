@@ -1,0 +1,105 @@
+// Device-side pieces shared by the contraction kernels: storage element
+// types, the workgroup-to-tile map and the fused write-back
+// D = op_D(alpha * tile + beta * op_C(C)) (crates/tprims-kernel/src/pack/writeback.rs:114-300).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kargs.h"
+
+namespace tp {
+namespace dev {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) Cx {
+  T re, im;
+};
+
+// Storage element of a dtype.
+template <typename T, bool CPLX>
+struct Store;
+template <typename T>
+struct Store<T, false> {
+  typedef T S;
+  __device__ static inline T re(S v) { return v; }
+  __device__ static inline T im(S) { return T(0); }
+  __device__ static inline S make(T r, T) { return r; }
+};
+template <typename T>
+struct Store<T, true> {
+  typedef Cx<T> S;
+  __device__ static inline T re(S v) { return v.re; }
+  __device__ static inline T im(S v) { return v.im; }
+  __device__ static inline S make(T r, T i) { return S{r, i}; }
+};
+
+// Workgroup id -> (h, tile_m, tile_n). Consecutive hardware ids go round-robin
+// over the 8 XCDs; remapping gives each XCD a contiguous run of logical tiles,
+// and the logical order walks groups of 8 tile-rows so neighbours share A rows
+// and B columns in that XCD's L2.
+__device__ inline void decode_tile(int64_t bid, int64_t nwg, int tiles_m, int tiles_n, int64_t& h, int& tm, int& tn) {
+  if ((nwg & 7) == 0) bid = (bid & 7) * (nwg >> 3) + (bid >> 3);
+  const int64_t per_h = (int64_t)tiles_m * tiles_n;
+  h = bid / per_h;
+  int t = (int)(bid - h * per_h);
+  const int GM = 8;
+  int group = t / (GM * tiles_n);
+  int first_m = group * GM;
+  int gsize = min(tiles_m - first_m, GM);
+  int r = t - group * GM * tiles_n;
+  tm = first_m + r % gsize;
+  tn = r / gsize;
+}
+
+// Writes a staged BM x BN tile (Dre/Dim, row pitch LDD, in LDS) to D along
+// D's contiguous axis: alpha scaling, beta * op_C(C) only when beta != 0 (C is
+// never read otherwise), op_D last, and the broadcast copies of an adjoint.
+template <typename T, typename TS, bool CPLX, int BM, int BN, int NT, int LDD>
+__device__ inline void write_back(const ContractArgs& p, int64_t h, int64_t m0, int64_t n0, const TS* Dre,
+                                  const TS* Dim) {
+  typedef Store<T, CPLX> St;
+  typedef typename St::S S;
+  const int tid = threadIdx.x;
+  S* __restrict__ D = (S*)p.D + p.hd[h];
+  const S* __restrict__ Cp = (const S*)p.C + p.hc[h];
+  const T alr = (T)p.alpha[0], ali = (T)p.alpha[1];
+  const T ber = (T)p.beta[0], bei = (T)p.beta[1];
+  const bool rc = p.read_c, cc = p.conj_c, cd = p.conj_d;
+  const int64_t M = p.M, N = p.N, R = p.R;
+  constexpr int EPI = BM * BN / NT;
+#pragma unroll 4
+  for (int t = 0; t < EPI; t++) {
+    const int e = tid + t * NT;
+    int ml, nl;
+    if (p.d_mmajor) {
+      ml = e % BM;
+      nl = e / BM;
+    } else {
+      ml = e / BN;
+      nl = e % BN;
+    }
+    const int64_t m = m0 + ml, n = n0 + nl;
+    if (m >= M || n >= N) continue;
+    T xr = (T)Dre[ml * LDD + nl];
+    T xi = CPLX ? (T)Dim[ml * LDD + nl] : T(0);
+    T vr = alr * xr - ali * xi;
+    T vi = alr * xi + ali * xr;
+    if (rc) {
+      S cv = Cp[p.cm[m] + p.cn[n]];
+      T cr = St::re(cv), ci = St::im(cv);
+      if (cc) ci = -ci;
+      vr += ber * cr - bei * ci;
+      vi += ber * ci + bei * cr;
+    }
+    if (cd) vi = -vi;
+    const S out = St::make(vr, vi);
+    const int64_t off = p.dm[m] + p.dn[n];
+    if (p.dr == nullptr) {
+      D[off] = out;
+    } else {
+      for (int64_t r = 0; r < R; r++) D[off + p.dr[r]] = out;
+    }
+  }
+}
+
+}  // namespace dev
+}  // namespace tp

@@ -128,7 +128,7 @@ int choose_tile(const Plan& p) {
   if (prec != PREC_STORAGE) {
     cands = {{STILE_128x128, 1.15}, {STILE_64x64, 1.0}};
   } else {
-    cands = {{TILE_128x64, 1.1}, {TILE_64x64, 1.0}, {TILE_64x32, 0.85}, {TILE_32x32, 0.7}};
+    cands = {{TILE_128x64, 1.1}, {TILE_64x64, 1.0}, {TILE_32x32, 0.7}};
   }
   int best = 0;
   double best_score = -1;

@@ -80,7 +80,7 @@ def test_host_operands_are_staged(gpu, name, dtype):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tile", [1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [1, 2, 3])
 @pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
 def test_every_tile(gpu, tile, dtype):
     if tile == 3 and dtype == tp.DType.C64:
