@@ -112,7 +112,7 @@ Axis scaled(const Axis& a, int64_t extent, int64_t by) {
 }
 
 bool tile_allowed(int id, DType t, int prec) {
-  if (prec != PREC_STORAGE) return id == STILE_128x128 || id == STILE_64x64;
+  if (prec != PREC_STORAGE) return id == STILE_64x64 || (id == STILE_128x128 && !is_complex(t));
   if (id >= 10) return false;
   if (id == TILE_128x64 && t == DType::C64) return false;  // staging would exceed 64 KiB
   return tile_cfg(id) != nullptr;
