@@ -105,6 +105,13 @@ TAPP_EXPORT TAPP_error tprims_group_use_graph(tprims_group group, TAPP_executor 
 /* Launch every item separately (one kernel per item), for comparison. */
 TAPP_EXPORT TAPP_error tprims_group_execute_each(tprims_group group, TAPP_executor exec);
 TAPP_EXPORT TAPP_error tprims_group_info(tprims_group group, int64_t* tiles, int64_t* flops, int32_t* tile_id);
+/* Fix the tile shape of the persistent launch (a storage-precision tile id). */
+TAPP_EXPORT TAPP_error tprims_group_set_tile(tprims_group group, int32_t tile_id);
+/* The autotuner: times every storage tile shape on the group's own operands
+ * (reps launches each, HIP events on the executor's stream), keeps the fastest
+ * and reports it. Operands are overwritten as by tprims_group_execute. */
+TAPP_EXPORT TAPP_error tprims_group_autotune(tprims_group group, TAPP_executor exec, int reps, int32_t* best_tile,
+                                             double* best_ms);
 TAPP_EXPORT TAPP_error tprims_group_destroy(tprims_group group);
 
 /* ---- differentiation --------------------------------------------------------
@@ -119,6 +126,8 @@ TAPP_EXPORT TAPP_error tprims_group_destroy(tprims_group group);
 TAPP_EXPORT TAPP_error tprims_product_vjp(TAPP_tensor_product plan, TAPP_executor exec, const void* alpha,
                                           const void* A, const void* B, const void* dD, void* dA, void* dB,
                                           int accumulate);
+/* The plan of the adjoint contraction (which = 0: dA, 1: dB). */
+TAPP_EXPORT TAPP_error tprims_product_vjp_report(TAPP_tensor_product plan, int which, tprims_plan_report* out);
 
 #ifdef __cplusplus
 }

@@ -31,21 +31,24 @@ __global__ void k_output_pass(ContractArgs p) {
       m = r % p.M;
       h = r / p.M;
     }
-    T vr = 0, vi = 0;
-    if (p.read_c) {
-      S cv = ((const S*)p.C)[p.hc[h] + p.cm[m] + p.cn[n]];
-      T cr = St::re(cv), ci = St::im(cv);
-      if (p.conj_c) ci = -ci;
-      vr = ber * cr - bei * ci;
-      vi = ber * ci + bei * cr;
-    }
-    if (p.conj_d) vi = -vi;
-    const S out = St::make(vr, vi);
     S* D = (S*)p.D;
     const int64_t off = p.hd[h] + p.dm[m] + p.dn[n];
-    if (p.dr == nullptr) D[off] = out;
-    else
-      for (int64_t r = 0; r < p.R; r++) D[off + p.dr[r]] = out;
+    const int64_t coff = p.read_c ? p.hc[h] + p.cm[m] + p.cn[n] : 0;
+    // Broadcast copies (adjoints) read their own previous value: C is D there.
+    const int64_t copies = p.dr == nullptr ? 1 : p.R;
+    for (int64_t r = 0; r < copies; r++) {
+      const int64_t roff = p.dr == nullptr ? 0 : p.dr[r];
+      T vr = 0, vi = 0;
+      if (p.read_c) {
+        S cv = ((const S*)p.C)[coff + roff];
+        T cr = St::re(cv), ci = St::im(cv);
+        if (p.conj_c) ci = -ci;
+        vr = ber * cr - bei * ci;
+        vi = ber * ci + bei * cr;
+      }
+      if (p.conj_d) vi = -vi;
+      D[off + roff] = St::make(vr, vi);
+    }
   }
 }
 

@@ -154,6 +154,38 @@ int execute_items(Plan* p, TAPP_executor exec, const void* alpha, const void* be
 
 }  // namespace
 
+namespace tp {
+
+void fill_report(const Plan* p, tprims_plan_report* out) {
+  memset(out, 0, sizeof *out);
+  out->m = p->M;
+  out->n = p->N;
+  out->k = p->K;
+  out->batch = p->H;
+  out->flops = p->strategy == 0 ? p->flops() : 0;
+  out->dtype = (int32_t)p->prob.dtype;
+  out->complex_method = p->cfg.complex_method;
+  out->precision = p->cfg.precision;
+  out->tile = p->tile;
+  if (const TileCfg* t = tile_cfg(p->tile)) {
+    out->tile_m = t->bm;
+    out->tile_n = t->bn;
+    out->tile_k = t->bk;
+  }
+  out->a_k_major = p->a_kmajor;
+  out->b_k_major = p->b_kmajor;
+  out->d_m_major = p->d_mmajor;
+  out->m_axes = (int32_t)p->m_ax.size();
+  out->n_axes = (int32_t)p->n_ax.size();
+  out->k_axes = (int32_t)p->k_ax.size();
+  out->h_axes = (int32_t)p->h_ax.size();
+  out->strategy = p->strategy;
+  std::string k = p->kernel_name();
+  strncpy(out->kernel, k.c_str(), sizeof out->kernel - 1);
+}
+
+}  // namespace tp
+
 extern "C" {
 
 // ---- status (status.rs:69-94, 165-171) ----------------------------------------
@@ -313,32 +345,7 @@ TAPP_error TAPP_destroy_tensor_product(TAPP_tensor_product plan) {
 TAPP_error tprims_product_report(TAPP_tensor_product plan, tprims_plan_report* out) {
   return ffi([&] {
     if (plan == 0 || !out) fail(ERR_INVALID_ARGUMENT, "plan or report is null or zero");
-    Plan* p = (Plan*)plan;
-    memset(out, 0, sizeof *out);
-    out->m = p->M;
-    out->n = p->N;
-    out->k = p->K;
-    out->batch = p->H;
-    out->flops = p->strategy == 0 ? p->flops() : 0;
-    out->dtype = (int32_t)p->prob.dtype;
-    out->complex_method = p->cfg.complex_method;
-    out->precision = p->cfg.precision;
-    out->tile = p->tile;
-    if (const TileCfg* t = tile_cfg(p->tile)) {
-      out->tile_m = t->bm;
-      out->tile_n = t->bn;
-      out->tile_k = t->bk;
-    }
-    out->a_k_major = p->a_kmajor;
-    out->b_k_major = p->b_kmajor;
-    out->d_m_major = p->d_mmajor;
-    out->m_axes = (int32_t)p->m_ax.size();
-    out->n_axes = (int32_t)p->n_ax.size();
-    out->k_axes = (int32_t)p->k_ax.size();
-    out->h_axes = (int32_t)p->h_ax.size();
-    out->strategy = p->strategy;
-    std::string k = p->kernel_name();
-    strncpy(out->kernel, k.c_str(), sizeof out->kernel - 1);
+    fill_report((Plan*)plan, out);
   });
 }
 

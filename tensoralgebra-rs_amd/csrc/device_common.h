@@ -81,22 +81,25 @@ __device__ inline void write_back(const ContractArgs& p, int64_t h, int64_t m0, 
     if (m >= M || n >= N) continue;
     T xr = (T)Dre[ml * LDD + nl];
     T xi = CPLX ? (T)Dim[ml * LDD + nl] : T(0);
-    T vr = alr * xr - ali * xi;
-    T vi = alr * xi + ali * xr;
-    if (rc) {
-      S cv = Cp[p.cm[m] + p.cn[n]];
-      T cr = St::re(cv), ci = St::im(cv);
-      if (cc) ci = -ci;
-      vr += ber * cr - bei * ci;
-      vi += ber * ci + bei * cr;
-    }
-    if (cd) vi = -vi;
-    const S out = St::make(vr, vi);
+    const T pr = alr * xr - ali * xi;
+    const T pi = alr * xi + ali * xr;
     const int64_t off = p.dm[m] + p.dn[n];
-    if (p.dr == nullptr) {
-      D[off] = out;
-    } else {
-      for (int64_t r = 0; r < R; r++) D[off + p.dr[r]] = out;
+    const int64_t coff = rc ? p.cm[m] + p.cn[n] : 0;
+    // A broadcast (adjoint) writes every copy; its C, when read, is D itself,
+    // so each copy accumulates its own previous value.
+    const int64_t copies = p.dr == nullptr ? 1 : R;
+    for (int64_t r = 0; r < copies; r++) {
+      const int64_t roff = p.dr == nullptr ? 0 : p.dr[r];
+      T vr = pr, vi = pi;
+      if (rc) {
+        S cv = Cp[coff + roff];
+        T cr = St::re(cv), ci = St::im(cv);
+        if (cc) ci = -ci;
+        vr += ber * cr - bei * ci;
+        vi += ber * ci + bei * cr;
+      }
+      if (cd) vi = -vi;
+      D[off + roff] = St::make(vr, vi);
     }
   }
 }

@@ -1,23 +1,340 @@
-// Grouped execution and differentiation entry points (placeholders).
+// Grouped execution: many independent contractions of one dtype, each with its
+// own plan and shape, run as ONE persistent kernel over a device work list
+// (the GPU counterpart of crates/tprims-contract/src/batch.rs:125
+// contract_batched, which validates all items before running any, and of
+// crates/tprims-capi/src/execute.rs:235 TAPP_execute_batched_product).
+//
+// The work list is one record per item (its kernel arguments and where its
+// tiles start in the flattened tile space); workgroups stride over the tile
+// space, so items of very different size share one launch. Items are ordered
+// by descending K (the cost of one tile) so the longest tiles start first.
+// The tile shape of the launch is chosen by measurement (autotune) among the
+// storage-precision tiles; the launch can be captured once into a hipGraph and
+// replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
 #include "capi_util.h"
 #include "executor.h"
+#include "kernels.h"
+
 extern "C" {
 #include "tprims/tprims.h"
 }
+
 using namespace tp;
+
+namespace {
+
+struct Group {
+  Executor* ex = nullptr;
+  DType dtype = DType::F64;
+  int method = 0;
+  int tile = TILE_64x64;
+  int grid = 0;
+  int64_t total_tiles = 0;
+  int64_t flops = 0;
+  std::vector<Plan*> plans;
+  std::vector<ContractArgs> args;   // per contraction item, tile fields unset
+  std::vector<int> own_tile;        // the tile each item's own plan chose
+  std::vector<ContractArgs> passes; // items that only need the output pass
+  GroupItem* d_items = nullptr;
+  int n_items = 0;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t capture = nullptr;  // the legacy null stream cannot be captured
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+  ~Group() {
+    drop_graph();
+    if (capture) (void)hipStreamDestroy(capture);
+    if (d_items) (void)hipFree(d_items);
+  }
+};
+
+void read_scalars(const void* p, DType t, int i, double out[2]) {
+  out[0] = out[1] = 0;
+  switch (t) {
+    case DType::F32: out[0] = ((const float*)p)[i]; break;
+    case DType::F64: out[0] = ((const double*)p)[i]; break;
+    case DType::C32:
+      out[0] = ((const float*)p)[2 * i];
+      out[1] = ((const float*)p)[2 * i + 1];
+      break;
+    case DType::C64:
+      out[0] = ((const double*)p)[2 * i];
+      out[1] = ((const double*)p)[2 * i + 1];
+      break;
+  }
+}
+
+bool device_pointer(const void* p, int device) {
+  if (!p) return true;
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return (at.type == hipMemoryTypeDevice && at.device == device) || at.type == hipMemoryTypeManaged;
+}
+
+// Builds the device work list for the group's current tile.
+void build_work_list(Group& g) {
+  const TileCfg* t = tile_cfg(g.tile);
+  std::vector<GroupItem> items;
+  int64_t total = 0;
+  for (const ContractArgs& a : g.args) {
+    GroupItem it{};
+    it.args = a;
+    it.args.tiles_m = (int)((a.M + t->bm - 1) / t->bm);
+    it.args.tiles_n = (int)((a.N + t->bn - 1) / t->bn);
+    it.tile_begin = total;
+    total += (int64_t)it.args.tiles_m * it.args.tiles_n * a.H;
+    items.push_back(it);
+  }
+  g.drop_graph();
+  if (g.d_items) {
+    hip_check(hipFree(g.d_items), "hipFree(work list)");
+    g.d_items = nullptr;
+  }
+  g.n_items = (int)items.size();
+  g.total_tiles = total;
+  if (!items.empty()) {
+    hip_check(hipMalloc((void**)&g.d_items, items.size() * sizeof(GroupItem)), "hipMalloc(work list)");
+    hip_check(hipMemcpy(g.d_items, items.data(), items.size() * sizeof(GroupItem), hipMemcpyHostToDevice),
+              "uploading the work list");
+  }
+  // Persistent grid: enough workgroups to fill every CU a few times over.
+  int cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, g.ex->device) == hipSuccess) cus = prop.multiProcessorCount;
+  else (void)hipGetLastError();
+  const int per_cu = t->threads >= 256 ? 2 : 8;
+  g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)cus * per_cu));
+}
+
+void launch_all(Group& g, hipStream_t s) {
+  for (const ContractArgs& a : g.passes) hip_check(launch_output_pass(a, g.dtype, s), "output-pass launch");
+  if (g.n_items > 0)
+    hip_check(launch_grouped(g.d_items, g.n_items, g.total_tiles, g.dtype, g.method, g.tile, g.grid, s),
+              "grouped launch");
+}
+
+void run_group(Group& g, Executor& ex) {
+  DeviceGuard dg(ex.device);
+  hipStream_t s = ex.get_stream();
+  if (g.use_graph) {
+    if (!g.gexec) {
+      // Capture on a private stream (the executor's may be the null stream);
+      // the graph is then launched on the executor's stream.
+      if (!g.capture) hip_check(hipStreamCreateWithFlags(&g.capture, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipStreamBeginCapture(g.capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        launch_all(g, g.capture);
+      } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(g.capture, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(g.capture, &g.graph), "hipStreamEndCapture");
+      hip_check(hipGraphInstantiate(&g.gexec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    hip_check(hipGraphLaunch(g.gexec, s), "hipGraphLaunch");
+  } else {
+    launch_all(g, s);
+  }
+  if (!ex.async.load()) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+Group* group_of(tprims_group h) {
+  if (h == 0) fail(ERR_INVALID_ARGUMENT, "group is null or zero");
+  return (Group*)h;
+}
+
+}  // namespace
+
 extern "C" {
-TAPP_error tprims_group_create(tprims_group*, TAPP_executor, int, const TAPP_tensor_product*, const void*,
-                               const void* const*, const void* const*, const void*, const void* const*,
-                               void* const*) {
-  return ffi([] { fail(ERR_UNSUPPORTED, "grouped execution is not available yet"); });
+
+TAPP_error tprims_group_create(tprims_group* out, TAPP_executor exec, int num_items, const TAPP_tensor_product* plans,
+                               const void* alpha, const void* const* A, const void* const* B, const void* beta,
+                               const void* const* C, void* const* D) {
+  return ffi([&] {
+    if (!out) fail(ERR_INVALID_ARGUMENT, "group out-parameter is null");
+    if (num_items < 0) fail(ERR_SHAPE, "negative item count");
+    if (num_items > 0 && (!plans || !alpha || !beta || !A || !B || !D))
+      fail(ERR_INVALID_ARGUMENT, "a plan, scalar or pointer array is null");
+    std::unique_ptr<Group> g(new Group);
+    Executor* ex = resolve_executor(exec);
+    g->ex = ex;
+    // Validate every item before anything is allocated on the device.
+    for (int i = 0; i < num_items; i++) {
+      if (plans[i] == 0) fail(ERR_INVALID_ARGUMENT, "plan " + std::to_string(i) + " is null");
+      Plan* p = (Plan*)plans[i];
+      if (i == 0) {
+        g->dtype = p->prob.dtype;
+        g->method = p->cfg.complex_method;
+      }
+      if (p->prob.dtype != g->dtype) fail(ERR_DTYPE, "grouped items must share one dtype");
+      if (p->cfg.complex_method != g->method) fail(ERR_UNSUPPORTED, "grouped items must share one complex method");
+      if (p->cfg.precision != PREC_STORAGE) fail(ERR_UNSUPPORTED, "grouped items run at the storage precision");
+      double al[2], be[2];
+      read_scalars(alpha, g->dtype, i, al);
+      read_scalars(beta, g->dtype, i, be);
+      const void* c = C ? C[i] : nullptr;
+      validate_item(*p, Item{A[i], B[i], c, D[i]}, be[0] == 0 && be[1] == 0);
+      for (const void* q : {A[i], B[i], c, (const void*)D[i]})
+        if (!device_pointer(q, ex->device)) fail(ERR_DEVICE, "grouped operands must be device memory of the executor");
+    }
+    DeviceGuard dg(ex->device);
+    std::vector<std::pair<int64_t, int>> order;
+    for (int i = 0; i < num_items; i++) order.push_back({((Plan*)plans[i])->K, i});
+    std::stable_sort(order.begin(), order.end(), [](auto& x, auto& y) { return x.first > y.first; });
+    for (auto& [k, i] : order) {
+      Plan* p = (Plan*)plans[i];
+      g->plans.push_back(p);
+      if (p->strategy == 2) continue;
+      double al[2], be[2];
+      read_scalars(alpha, g->dtype, i, al);
+      read_scalars(beta, g->dtype, i, be);
+      const void* c = C ? C[i] : nullptr;
+      if (!c) be[0] = be[1] = 0;
+      const bool read_c = be[0] != 0 || be[1] != 0;
+      ContractArgs a = p->args(ex->device, A[i], B[i], c, D[i], al, be, read_c);
+      if (p->strategy == 1 || (al[0] == 0 && al[1] == 0)) {
+        g->passes.push_back(a);
+      } else {
+        g->args.push_back(a);
+        g->own_tile.push_back(p->tile);
+        g->flops += p->flops();
+      }
+    }
+    // Default tile: the one most items' plans chose.
+    int votes[16] = {0};
+    for (Plan* p : g->plans)
+      if (p->tile > 0 && p->tile < 4) votes[p->tile]++;
+    g->tile = (int)(std::max_element(votes, votes + 4) - votes);
+    if (g->tile == 0 || (g->tile == TILE_128x64 && g->dtype == DType::C64)) g->tile = TILE_64x64;
+    build_work_list(*g);
+    *out = (tprims_group)g.release();
+  });
 }
-TAPP_error tprims_group_execute(tprims_group, TAPP_executor) { return ffi([] { fail(ERR_UNSUPPORTED, "n/a"); }); }
-TAPP_error tprims_group_use_graph(tprims_group, TAPP_executor, int) { return ffi([] { fail(ERR_UNSUPPORTED, "n/a"); }); }
-TAPP_error tprims_group_execute_each(tprims_group, TAPP_executor) { return ffi([] { fail(ERR_UNSUPPORTED, "n/a"); }); }
-TAPP_error tprims_group_info(tprims_group, int64_t*, int64_t*, int32_t*) { return ffi([] { fail(ERR_UNSUPPORTED, "n/a"); }); }
-TAPP_error tprims_group_destroy(tprims_group) { return ffi([] { fail(ERR_UNSUPPORTED, "n/a"); }); }
-TAPP_error tprims_product_vjp(TAPP_tensor_product, TAPP_executor, const void*, const void*, const void*, const void*,
-                              void*, void*, int) {
-  return ffi([] { fail(ERR_UNSUPPORTED, "n/a"); });
+
+TAPP_error tprims_group_execute(tprims_group group, TAPP_executor exec) {
+  return ffi([&] {
+    Group* g = group_of(group);
+    Executor* ex = exec ? (Executor*)exec : g->ex;
+    if (ex->device != g->ex->device) fail(ERR_DEVICE, "the group was built for another device");
+    ex->inflight++;
+    struct Release {
+      Executor* e;
+      ~Release() { e->inflight--; }
+    } rel{ex};
+    run_group(*g, *ex);
+  });
 }
+
+TAPP_error tprims_group_use_graph(tprims_group group, TAPP_executor, int enable) {
+  return ffi([&] {
+    Group* g = group_of(group);
+    g->use_graph = enable != 0;
+    if (!g->use_graph) g->drop_graph();
+  });
 }
+
+TAPP_error tprims_group_execute_each(tprims_group group, TAPP_executor exec) {
+  return ffi([&] {
+    Group* g = group_of(group);
+    Executor* ex = exec ? (Executor*)exec : g->ex;
+    DeviceGuard dg(ex->device);
+    hipStream_t s = ex->get_stream();
+    for (const ContractArgs& a : g->passes) hip_check(launch_output_pass(a, g->dtype, s), "output-pass launch");
+    // Each item on the tile its own plan chose.
+    for (size_t i = 0; i < g->args.size(); i++)
+      hip_check(launch_contract(g->args[i], g->dtype, g->method, g->own_tile[i], s), "contraction launch");
+    if (!ex->async.load()) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  });
+}
+
+TAPP_error tprims_group_info(tprims_group group, int64_t* tiles, int64_t* flops, int32_t* tile_id) {
+  return ffi([&] {
+    Group* g = group_of(group);
+    if (tiles) *tiles = g->total_tiles;
+    if (flops) *flops = g->flops;
+    if (tile_id) *tile_id = g->tile;
+  });
+}
+
+TAPP_error tprims_group_set_tile(tprims_group group, int32_t tile_id) {
+  return ffi([&] {
+    Group* g = group_of(group);
+    const TileCfg* t = tile_cfg(tile_id);
+    if (!t || tile_id >= 10 || (tile_id == TILE_128x64 && g->dtype == DType::C64))
+      fail(ERR_UNSUPPORTED, "tile id not available for grouped execution");
+    DeviceGuard dg(g->ex->device);
+    g->tile = tile_id;
+    build_work_list(*g);
+  });
+}
+
+TAPP_error tprims_group_autotune(tprims_group group, TAPP_executor exec, int reps, int32_t* best_tile,
+                                 double* best_ms) {
+  return ffi([&] {
+    Group* g = group_of(group);
+    Executor* ex = exec ? (Executor*)exec : g->ex;
+    if (reps < 1) reps = 1;
+    DeviceGuard dg(ex->device);
+    hipStream_t s = ex->get_stream();
+    const bool graph = g->use_graph;
+    g->use_graph = false;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    int best = g->tile;
+    float best_t = 1e30f;
+    for (int id : {TILE_64x64, TILE_32x32, TILE_128x64}) {
+      if (id == TILE_128x64 && g->dtype == DType::C64) continue;
+      g->tile = id;
+      build_work_list(*g);
+      launch_all(*g, s);  // warm-up
+      hip_check(hipEventRecord(e0, s), "hipEventRecord");
+      for (int r = 0; r < reps; r++) launch_all(*g, s);
+      hip_check(hipEventRecord(e1, s), "hipEventRecord");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      ms /= reps;
+      if (ms < best_t) {
+        best_t = ms;
+        best = id;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g->tile = best;
+    build_work_list(*g);
+    g->use_graph = graph;
+    if (best_tile) *best_tile = best;
+    if (best_ms) *best_ms = best_t;
+  });
+}
+
+TAPP_error tprims_group_destroy(tprims_group group) {
+  return ffi([&] {
+    Group* g = group_of(group);
+    delete g;
+  });
+}
+
+}  // extern "C"

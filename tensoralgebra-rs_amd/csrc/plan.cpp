@@ -351,6 +351,8 @@ std::string Plan::kernel_name() const {
 }
 
 Plan::~Plan() {
+  delete adj[0];
+  delete adj[1];
   for (auto& [d, s] : dev) {
     int prev = 0;
     (void)hipGetDevice(&prev);

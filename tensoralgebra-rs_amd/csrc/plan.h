@@ -48,6 +48,10 @@ struct Plan {
   std::mutex mu;
   std::map<int, DeviceScatter> dev;
 
+  // Adjoint plans (cotangents of A and B), built on the first VJP.
+  Plan* adj[2] = {nullptr, nullptr};
+  bool adj_zero_fill[2] = {false, false};
+
   int64_t flops() const { return (is_complex(prob.dtype) ? 8 : 2) * M * N * K * H; }
   const DeviceScatter& on_device(int device);
   // Kernel arguments for data pointers (device addresses) and scalars.

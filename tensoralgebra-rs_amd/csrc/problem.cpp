@@ -70,7 +70,12 @@ Span span_of(DType t, const Layout& l) {
   return sp;
 }
 
-bool injective_modes(std::vector<std::pair<int64_t, int64_t>> axes /* extent, stride */) {
+// validate.rs:122-143: after sorting by |stride|, each stride must clear the
+// reach of the smaller ones, reach = stride * extent (conservative). With
+// `tight`, the reach is the span actually addressed, 1 + sum (extent-1)|stride|,
+// which still proves injectivity and also admits diagonal outputs followed by
+// further axes (the adjoints write such diagonals).
+bool injective_modes(std::vector<std::pair<int64_t, int64_t>> axes /* extent, stride */, bool tight = false) {
   for (auto& a : axes)
     if (a.first == 0) return true;
   std::vector<std::pair<__int128, __int128>> v;
@@ -80,7 +85,7 @@ bool injective_modes(std::vector<std::pair<int64_t, int64_t>> axes /* extent, st
   __int128 reach = 1;
   for (auto& [s, e] : v) {
     if (s < reach) return false;
-    reach = s * e;
+    reach = tight ? reach + s * (e - 1) : s * e;
   }
   return true;
 }
@@ -178,7 +183,8 @@ Problem lower(DType dtype, const Operand& a, const std::vector<int64_t>& la, con
   {
     std::vector<std::pair<int64_t, int64_t>> v;
     for (auto& m : rd) v.push_back({m.extent, m.stride});
-    if (!injective_modes(v)) fail(ERR_ALIASED, "the output addresses one element through two index tuples");
+    if (!injective_modes(v, allow_broadcast))
+      fail(ERR_ALIASED, "the output addresses one element through two index tuples");
   }
 
   if (c) {

@@ -59,8 +59,10 @@ struct Problem {
   int64_t macs() const;  // product of every M, N, K, H extent
 };
 
-// The lowering (validate.rs:207-308). allow_broadcast admits output-only labels
-// into roles.r instead of refusing them with ERR_UNSUPPORTED.
+// The lowering (validate.rs:207-308). allow_broadcast (adjoint problems only)
+// admits output-only labels into roles.r instead of refusing them with
+// ERR_UNSUPPORTED, and proves output injectivity with the addressed span
+// instead of the reference's coarser stride*extent reach.
 Problem lower(DType dtype, const Operand& a, const std::vector<int64_t>& la, const Operand& b,
               const std::vector<int64_t>& lb, const Operand* c, const std::vector<int64_t>* lc,
               const Operand& d, const std::vector<int64_t>& ld, bool allow_broadcast = false);

@@ -389,6 +389,36 @@ class Plan:
             c = d
         self.execute_raw(alpha, _addr(a), _addr(b), beta, _addr(c), _addr(d), executor)
 
+    def execute_dlpack(self, alpha, a, b, beta, c, d, executor=None):
+        """Execute with DLPack operands (tprims_execute_product_dlpack); each of
+        a, b, c (or None), d is a torch tensor exported through DLPack."""
+        from .dlpack import borrow
+
+        dt = self.problem.dtype
+        pa, _ka = _scalar(alpha, dt)
+        pb, _kb = _scalar(beta, dt)
+        keep = []
+        ta, tb, td = borrow(a, keep), borrow(b, keep), borrow(d, keep)
+        tc = borrow(c, keep) if c is not None else _capi.TprimsTensor()
+        ex = executor.handle if executor is not None else 0
+        check(lib().tprims_execute_product_dlpack(self.handle, ex, pa, ta, tb, pb, tc, td))
+
+    def vjp(self, alpha, a, b, dd, da=None, db=None, accumulate=False, executor=None):
+        """Cotangents dA, dB of D with respect to A and B (tprims_product_vjp);
+        operands are device tensors or addresses in the plan's layouts."""
+        dt = self.problem.dtype
+        pa, _ka = _scalar(alpha, dt)
+        ex = executor.handle if executor is not None else 0
+        check(lib().tprims_product_vjp(self.handle, ex, pa, _addr(a), _addr(b), _addr(dd), _addr(da), _addr(db),
+                                       int(bool(accumulate))))
+
+    def vjp_report(self, which):
+        r = _capi.PlanReportC()
+        check(lib().tprims_product_vjp_report(self.handle, int(which), ctypes.byref(r)))
+        out = {f: getattr(r, f) for f, _ in _capi.PlanReportC._fields_}
+        out["kernel"] = r.kernel.decode()
+        return out
+
     def close(self):
         if getattr(self, "handle", 0):
             lib().TAPP_destroy_tensor_product(self.handle)
@@ -420,3 +450,75 @@ def contract(labels_a, a, labels_b, b, labels_d, d, alpha=1.0, beta=0.0, c=None,
     plan = Plan(Problem.from_labels(dt, spec(a, conj_a), spec(b, conj_b), cs, spec(d, conj_d), lab), config)
     plan.execute_raw(alpha, _addr(a), _addr(b), beta, _addr(c), _addr(d), executor)
     return plan
+
+
+class Group:
+    """Many contractions of one dtype run as one persistent kernel
+    (tprims_group_create; the GPU form of contract_batched,
+    crates/tprims-contract/src/batch.rs:125). Operands are device tensors."""
+
+    def __init__(self, plans, alphas, a, b, betas, c, d, executor=None):
+        n = len(plans)
+        dt = plans[0].problem.dtype
+        scal = (ctypes.c_double if dt in (DType.F64, DType.C64) else ctypes.c_float)
+        w = 2 if dt.is_complex else 1
+
+        def scalars(vs):
+            flat = []
+            for v in vs:
+                v = complex(v)
+                flat += [v.real, v.imag][:w]
+            return (scal * max(1, len(flat)))(*flat)
+
+        def ptrs(xs):
+            return (ctypes.c_void_p * max(1, n))(*[_addr(x) for x in xs])
+
+        self._keep = [scalars(alphas), scalars(betas), ptrs(a), ptrs(b), ptrs(c) if c is not None else None, ptrs(d)]
+        self.plans = plans
+        h = _capi.intptr(0)
+        ex = executor.handle if executor is not None else 0
+        hs = (_capi.intptr * max(1, n))(*[p.handle for p in plans])
+        self._keep.append(hs)
+        check(lib().tprims_group_create(ctypes.byref(h), ex, n, hs, ctypes.cast(self._keep[0], ctypes.c_void_p),
+                                        self._keep[2], self._keep[3], ctypes.cast(self._keep[1], ctypes.c_void_p),
+                                        self._keep[4], self._keep[5]))
+        self.handle = h.value
+        self.executor = executor
+
+    def _ex(self, executor):
+        e = executor or self.executor
+        return e.handle if e is not None else 0
+
+    def execute(self, executor=None):
+        check(lib().tprims_group_execute(self.handle, self._ex(executor)))
+
+    def execute_each(self, executor=None):
+        check(lib().tprims_group_execute_each(self.handle, self._ex(executor)))
+
+    def use_graph(self, enable=True, executor=None):
+        check(lib().tprims_group_use_graph(self.handle, self._ex(executor), int(bool(enable))))
+
+    def set_tile(self, tile):
+        check(lib().tprims_group_set_tile(self.handle, int(tile)))
+
+    def autotune(self, reps=3, executor=None):
+        t, ms = ctypes.c_int32(0), ctypes.c_double(0)
+        check(lib().tprims_group_autotune(self.handle, self._ex(executor), int(reps), ctypes.byref(t),
+                                          ctypes.byref(ms)))
+        return t.value, ms.value
+
+    def info(self):
+        tiles, flops, tile = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int32(0)
+        check(lib().tprims_group_info(self.handle, ctypes.byref(tiles), ctypes.byref(flops), ctypes.byref(tile)))
+        return {"tiles": tiles.value, "flops": flops.value, "tile": tile.value}
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            lib().tprims_group_destroy(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

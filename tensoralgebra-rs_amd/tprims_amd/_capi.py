@@ -89,8 +89,8 @@ EXPORTED = [
     "tprims_tapp_executor_create_hip", "tprims_tapp_executor_set_async", "tprims_tapp_executor_synchronize",
     "tprims_tapp_executor_get_stream", "tprims_tapp_executor_get_device", "tprims_create_tensor_product_ex",
     "tprims_product_report", "tprims_execute_product_dlpack", "tprims_group_create", "tprims_group_execute",
-    "tprims_group_use_graph", "tprims_group_execute_each", "tprims_group_info", "tprims_group_destroy",
-    "tprims_product_vjp",
+    "tprims_group_use_graph", "tprims_group_execute_each", "tprims_group_info", "tprims_group_set_tile",
+    "tprims_group_autotune", "tprims_group_destroy", "tprims_product_vjp", "tprims_product_vjp_report",
 ]
 
 
@@ -150,6 +150,10 @@ def _declare(lib):
         "tprims_group_info": (c_int, [intptr, i64p, i64p, ctypes.POINTER(ctypes.c_int32)]),
         "tprims_group_destroy": (c_int, [intptr]),
         "tprims_product_vjp": (c_int, [intptr, intptr, vp, vp, vp, vp, vp, vp, c_int]),
+        "tprims_product_vjp_report": (c_int, [intptr, c_int, ctypes.POINTER(PlanReportC)]),
+        "tprims_group_set_tile": (c_int, [intptr, ctypes.c_int32]),
+        "tprims_group_autotune": (c_int, [intptr, intptr, c_int, ctypes.POINTER(ctypes.c_int32),
+                                          ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,136 @@
+"""GPU tests of grouped execution (one persistent kernel over many uneven
+contractions, optionally captured in a hipGraph) against the FP64 oracle and
+against one launch per contraction."""
+import numpy as np
+import pytest
+
+import oracle
+import tprims_amd as tp
+from harness import NP, T, make_plan, tol
+
+
+def mps_items(dtype, n, seed, dims=(16, 32, 48), phys=(2, 4)):
+    """E[a,s,b] = sum_{c,t} L[a,c] * M[c,t,s,b] with uneven bond dims."""
+    rng = np.random.default_rng(seed)
+    items = []
+    for _ in range(n):
+        a, b, c = (int(rng.choice(dims)) for _ in range(3))
+        s, t = (int(rng.choice(phys)) for _ in range(2))
+        L = T([a, c], tp.col_major([a, c]), [1, 3])
+        Mt = T([c, t, s, b], tp.col_major([c, t, s, b]), [3, 4, 2, 5])
+        E = T([a, s, b], tp.col_major([a, s, b]), [1, 2, 5])
+        def draw(k):
+            x = rng.standard_normal(k) + (1j * rng.standard_normal(k) if dtype.is_complex else 0)
+            return x.astype(NP[dtype])
+        items.append((L, draw(a * c), Mt, draw(c * t * s * b), E, draw(a * s * b)))
+    return items
+
+
+def build(dtype, items, gpu, beta=0.0):
+    import torch
+
+    plans, A, B, C, D, want = [], [], [], [], [], []
+    alphas, betas = [], []
+    for k, (L, lv, Mt, mv, E, cv) in enumerate(items):
+        plans.append(make_plan(dtype, L, Mt, E))
+        A.append(torch.from_numpy(lv).to(gpu))
+        B.append(torch.from_numpy(mv).to(gpu))
+        C.append(torch.from_numpy(cv).to(gpu))
+        D.append(torch.zeros(len(cv), dtype=A[-1].dtype, device=gpu))
+        al = complex(1.0 + 0.1 * k, -0.2) if dtype.is_complex else 1.0 + 0.1 * k
+        alphas.append(al)
+        betas.append(beta)
+        want.append(oracle.contract(al, L.oracle_op(lv), Mt.oracle_op(mv), beta, E.oracle_op(cv),
+                                    E.oracle_op(np.zeros(len(cv), NP[dtype])), cplx=dtype.is_complex))
+    return plans, alphas, betas, A, B, C, D, want
+
+
+def check_all(dtype, D, want, label):
+    for k, (d, w) in enumerate(zip(D, want)):
+        err = oracle.max_rel_err(d.cpu().numpy().astype(np.complex128), w)
+        assert err <= tol(dtype), "%s item %d: %.3e" % (label, k, err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [tp.DType.C64, tp.DType.F64, tp.DType.F32, tp.DType.C32], ids=lambda d: d.name)
+def test_group_matches_oracle_and_per_item_launches(gpu, dtype):
+    import torch
+
+    items = mps_items(dtype, 24, 11)
+    plans, al, be, A, B, C, D, want = build(dtype, items, gpu, beta=0.5)
+    g = tp.Group(plans, al, A, B, be, C, D)
+    g.execute()
+    torch.cuda.synchronize()
+    check_all(dtype, D, want, "grouped")
+    grouped = [d.clone() for d in D]
+    for d in D:
+        d.zero_()
+    g.execute_each()
+    torch.cuda.synchronize()
+    check_all(dtype, D, want, "per-item")
+    info = g.info()
+    assert info["tiles"] > 0 and info["flops"] == sum(p.report()["flops"] for p in plans)
+    # K is never split: the same tile shape gives the same bits.
+    for k, p in enumerate(plans):
+        if p.report()["tile"] == info["tile"]:
+            assert torch.equal(grouped[k], D[k])
+
+
+@pytest.mark.gpu
+def test_group_graph_replays_match_eager(gpu):
+    import torch
+
+    dtype = tp.DType.C64
+    items = mps_items(dtype, 16, 5)
+    plans, al, be, A, B, C, D, want = build(dtype, items, gpu)
+    g = tp.Group(plans, al, A, B, be, C, D)
+    g.execute()
+    torch.cuda.synchronize()
+    eager = [d.clone() for d in D]
+    g.use_graph(True)
+    for rep in range(3):
+        for d in D:
+            d.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.execute()
+        torch.cuda.synchronize()
+        for k in range(len(D)):
+            assert torch.equal(D[k], eager[k]), "replay %d item %d" % (rep, k)
+        # eager work between replays
+        _ = (A[0] * 2).sum().item()
+    check_all(dtype, D, want, "graph")
+
+
+@pytest.mark.gpu
+def test_group_autotune_keeps_results(gpu):
+    import torch
+
+    dtype = tp.DType.C64
+    plans, al, be, A, B, C, D, want = build(dtype, mps_items(dtype, 12, 9), gpu)
+    g = tp.Group(plans, al, A, B, be, C, D)
+    tile, ms = g.autotune(reps=2)
+    assert tile in (1, 2) and ms > 0
+    for d in D:
+        d.zero_()
+    g.execute()
+    torch.cuda.synchronize()
+    check_all(dtype, D, want, "autotuned")
+    for t in (1, 2):
+        g.set_tile(t)
+        g.execute()
+        torch.cuda.synchronize()
+        check_all(dtype, D, want, "tile %d" % t)
+
+
+def test_group_rejects_mixed_dtypes_without_a_gpu_call():
+    a = T([4, 3], [1, 4], [0, 2])
+    b = T([3, 5], [1, 3], [2, 1])
+    d = T([4, 5], [1, 4], [0, 1])
+    p1 = make_plan(tp.DType.F64, a, b, d)
+    p2 = make_plan(tp.DType.F32, a, b, d)
+    import ctypes
+    bufs = [np.zeros(32) for _ in range(4)]
+    with pytest.raises(tp.TprimsError) as e:
+        tp.Group([p1, p2], [1.0, 1.0], [bufs[0], bufs[0]], [bufs[1], bufs[1]], [0.0, 0.0], None,
+                 [bufs[2], bufs[3]])
+    assert e.value.status in (3, 4)
