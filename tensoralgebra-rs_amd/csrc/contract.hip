@@ -73,11 +73,12 @@ struct Cfg {
   static constexpr int WAVES_M = BM / WM;
   static constexpr int NT = (BM / WM) * (BN / WN) * 64;
   static constexpr int PLANES = METHOD == M_REAL ? 1 : (METHOD == M_3M ? 3 : 2);
-  // Plane layouts: K-major gathers store [m][k] with pitch BK+4; M-major
-  // gathers store [k][m] with pitch BM+16 (both conflict-free for the
-  // 16-row x 4-k fragment reads).
-  static constexpr int LDAK = BK + 4, LDAM = BM + 16;
-  static constexpr int LDBK = BK + 4, LDBN = BN + 16;
+  // Plane layouts: K-major gathers store [m][k] with an odd pitch BK+1 (16
+  // consecutive rows then fall on distinct banks); M-major gathers store
+  // [k][m] with pitch BM+16. Both keep the 16-row x 4-k fragment reads and
+  // the gather stores free of bank conflicts.
+  static constexpr int LDAK = BK + 1, LDAM = BM + 16;
+  static constexpr int LDBK = BK + 1, LDBN = BN + 16;
   static constexpr int A_PLANE = AKM ? BM * LDAK : BK * LDAM;
   static constexpr int B_PLANE = BKM ? BN * LDBK : BK * LDBN;
   static constexpr int LDD = BN + 1;  // row pitch of the staging tile

@@ -55,9 +55,24 @@ char* Executor::ensure_scratch(size_t bytes) {
   return scratch;
 }
 
+char* Executor::ensure_panels(size_t bytes) {
+  if (bytes <= panels_cap) return panels;
+  if (panels) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    hip_check(hipFree(panels), "hipFree");
+    panels = nullptr;
+    panels_cap = 0;
+  }
+  size_t cap = bytes + bytes / 8;
+  hip_check(hipMalloc((void**)&panels, cap), "hipMalloc(panels)");
+  panels_cap = cap;
+  return panels;
+}
+
 Executor::~Executor() {
   if (stream_ready && stream) (void)hipStreamSynchronize(stream);
   if (scratch) (void)hipFree(scratch);
+  if (panels) (void)hipFree(panels);
   if (owns_stream && stream_ready && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -239,8 +254,10 @@ void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in
   if (!product) {
     hip_check(launch_output_pass(a, p.prob.dtype, s), "output-pass launch");
   } else if (p.cfg.precision == PREC_SPLIT_BF16_2 || p.cfg.precision == PREC_SPLIT_BF16_3) {
-    hip_check(launch_contract_split(a, p.prob.dtype, p.cfg.precision == PREC_SPLIT_BF16_2 ? 2 : 3, p.tile, s),
-              "split-BF16 contraction launch");
+    const int terms = p.cfg.precision == PREC_SPLIT_BF16_2 ? 2 : 3;
+    std::lock_guard<std::mutex> g(ex.panel_mu);
+    char* panels = ex.ensure_panels(split_panel_bytes(a, p.prob.dtype, terms, p.tile));
+    hip_check(launch_contract_split(a, p.prob.dtype, terms, p.tile, panels, s), "split-BF16 contraction launch");
   } else {
     hip_check(launch_contract(a, p.prob.dtype, p.cfg.complex_method, p.tile, s), "contraction launch");
   }

@@ -26,9 +26,14 @@ struct Executor {
   std::mutex mu;  // stream creation and scratch
   char* scratch = nullptr;
   size_t scratch_cap = 0;
+  // Split-BF16 operand panels (reused call after call in stream order).
+  std::mutex panel_mu;
+  char* panels = nullptr;
+  size_t panels_cap = 0;
 
   hipStream_t get_stream();
   char* ensure_scratch(size_t bytes);  // caller holds mu
+  char* ensure_panels(size_t bytes);   // caller holds panel_mu
   ~Executor();
 };
 

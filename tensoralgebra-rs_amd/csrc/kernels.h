@@ -32,8 +32,11 @@ const TileCfg* tile_cfg(int id);
 // The contraction kernel of a storage dtype, complex method and tile.
 // Returns hipSuccess or the launch error.
 hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s);
-// Split-BF16 kernels for f32/c32 storage (nterms 2 or 3).
-hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int tile_id, hipStream_t s);
+// Split-BF16 for f32/c32 storage (nterms 2 or 3): A and B are split once into
+// bf16 fragment panels (device scratch of split_panel_bytes), then multiplied.
+size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id);
+hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int tile_id, void* panels,
+                                 hipStream_t s);
 // D = op_D(beta * op_C(C)) over the output (empty K or alpha == 0).
 hipError_t launch_output_pass(const ContractArgs& a, DType t, hipStream_t s);
 // Zero every element a layout addresses (rank <= 8; used by the adjoints).

@@ -1,25 +1,32 @@
-// Split-BF16 contraction kernels for f32/c32 storage on gfx950's bf16 MFMA
+// Split-BF16 contraction for f32/c32 storage on gfx950's bf16 MFMA
 // (v_mfma_f32_16x16x32_bf16, fp32 accumulation). No counterpart exists in
 // the reference (crates/tprims-capi/src/product.rs:132 refuses any precision
-// other than the storage one); the gather, conjugation and write-back
-// semantics are those of the storage kernels.
+// other than the storage one); gather, conjugation and write-back semantics are
+// those of the storage kernels.
 //
-// Every f32 value x entering LDS is split on the fly, in registers, into
-// bf16 terms:  hi = bf16(x), r = x - hi (exact in f32), mid = bf16(r),
-// lo = bf16(r - mid). Nothing split is ever written to HBM.
+// Every f32 value x is split into bf16 terms:
+//   hi = bf16(x), r = x - hi (exact in f32), mid = bf16(r), lo = bf16(r - mid).
 //   2 terms (hi, lo = bf16(x - hi)):  a*b ~ ah*bh + ah*bl + al*bh           (3 MFMAs)
 //   3 terms (hi, mid, lo):            + am*bm + ah*bl + al*bh on top        (6 MFMAs)
 // The dropped products are below 2^-16 (2 terms) or 2^-24 (3 terms) of |a||b|.
-// A complex product takes four real ones (Re = rr - ii, Im = ri + ir), with
-// the imaginary-imaginary sum kept in its own accumulator and subtracted at
-// the end.
+// A complex product takes four real ones (Re = rr - ii, Im = ri + ir), the
+// imaginary-imaginary sum in its own accumulator, subtracted at the end.
 //
-// One workgroup computes a BM x BN tile over all of K in steps of 32 (one MFMA
-// depth). Each thread gathers runs of 4 consecutive k of one row (lanes along
-// K or along M/N, following the operand's contiguous axis), splits them and
-// stores each term plane row-major in k, so a lane's MFMA fragment (8 k of one
-// row) is one 16-byte LDS read. K offsets are loaded one step ahead of the
-// gathers, and the gathers of the next step overlap the MFMAs of this one.
+// Two stages, like the reference's pack-then-multiply driver
+// (crates/tprims-kernel/src/pack/pack.rs:152 packs through the scatter
+// vectors, with conjugation folded, before the microkernel runs):
+//  1. k_pack gathers each operand once through its scatter vectors (lanes
+//     along the operand's contiguous axis), splits it and writes bf16 panels
+//     in MFMA fragment order: for every (batch h, 32-wide K block, 16-row
+//     block, component, term) one 1 KiB fragment whose 16 bytes at lane*16
+//     are exactly what lane `lane` feeds the instruction (row lane%16,
+//     k 8*(lane/16) .. +7). Splitting once per element (not once per output
+//     tile) is what takes the split arithmetic off the MFMA loop.
+//  2. k_gemm streams the panels: per 32-wide K step a workgroup copies its
+//     row blocks (contiguous in the panel) into LDS with 16-byte loads, every
+//     fragment read is one conflict-free 16-byte LDS read, and the waves run
+//     3 or 6 bf16 MFMAs per real product. The next step's panels are loaded
+//     into registers while this step's MFMAs run.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -28,38 +35,14 @@
 namespace tp {
 namespace {
 
-using dev::Cx;
 using dev::Store;
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
-template <bool CPLX_, int TERMS_, int BM_, int BN_, int WM_, int WN_, bool AKM_, bool BKM_>
-struct SCfg {
-  static constexpr bool CPLX = CPLX_, AKM = AKM_, BKM = BKM_;
-  static constexpr int TERMS = TERMS_;
-  static constexpr int BM = BM_, BN = BN_, BK = 32, WM = WM_, WN = WN_;
-  static constexpr int WAVES_M = BM / WM;
-  static constexpr int NT = (BM / WM) * (BN / WN) * 64;
-  static constexpr int KV = 4;             // consecutive k per gathered run
-  static constexpr int KG = BK / KV;       // runs per row and slab
-  static constexpr int COMPS = CPLX ? 2 : 1;
-  static constexpr int LDK = BK + 8;       // bf16 pitch of a plane row (80 bytes)
-  static constexpr int A_PLANE = BM * LDK, B_PLANE = BN * LDK;
-  static constexpr int RUNS_A = BM * KG / NT, RUNS_B = BN * KG / NT;
-  static constexpr int KA = AKM ? KV : RUNS_A * KV;  // K offsets tracked per thread
-  static constexpr int KB = BKM ? KV : RUNS_B * KV;
-  static constexpr int RA = AKM ? RUNS_A : 1;        // row offsets tracked per thread
-  static constexpr int RB = BKM ? RUNS_B : 1;
-  static constexpr int MI = WM / 16, NI = WN / 16;
-  static constexpr int LDD = BN + 1;
-  static constexpr size_t LDS_MAIN = (size_t)COMPS * TERMS * (A_PLANE + B_PLANE) * 2;
-  static constexpr size_t LDS_EPI = (size_t)COMPS * BM * LDD * 4;
-  static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
-  static_assert(NT % KG == 0 && BM % (NT / KG) == 0 && BN % (NT / KG) == 0, "K-major run map");
-  static_assert(NT % BM == 0 && NT % BN == 0 && KG % (NT / BM) == 0 && KG % (NT / BN) == 0, "MN-major run map");
-};
+constexpr int FRAG = 512;  // bf16 per fragment (16 rows x 32 k)
+constexpr int KBLK = 32;
 
 // The bf16 terms of x (TERMS of them, leading first).
 template <int TERMS>
@@ -76,203 +59,180 @@ __device__ inline void split(float x, __bf16* t) {
   }
 }
 
-template <class C>
-__device__ inline void split_tile(const ContractArgs& p, int64_t h, int tm, int tn, char* smem) {
-  typedef Store<float, C::CPLX> St;
-  typedef typename St::S S;
-  constexpr int BM = C::BM, BN = C::BN, BK = C::BK, NT = C::NT, KV = C::KV, KG = C::KG;
-  constexpr int RUNS_A = C::RUNS_A, RUNS_B = C::RUNS_B;
+// Position of element (row rr in 0..15, k in 0..31) inside a fragment.
+__device__ inline int frag_pos(int rr, int k) { return ((k >> 3) * 16 + rr) * 8 + (k & 7); }
 
+struct PackArgs {
+  const void* X;
+  const int64_t* rv;  // row scatter (length R)
+  const int64_t* kv;  // K scatter (length K)
+  const int64_t* hv;  // batch scatter (length H)
+  int64_t R, K, H;
+  int64_t RB;  // row blocks (of 16) per (h, K block) in the panel, padded
+  int64_t KB;  // K blocks of 32
+  int32_t conj, kmajor;
+  __bf16* out;
+};
+
+// One workgroup packs 64 rows x 32 k of one batch entry.
+template <bool CPLX, int TERMS>
+__global__ __launch_bounds__(256) void k_pack(PackArgs p) {
+  typedef Store<float, CPLX> St;
+  typedef typename St::S S;
+  constexpr int COMPS = CPLX ? 2 : 1;
+  constexpr int PER = COMPS * TERMS;  // fragments per row block
+  __shared__ __attribute__((aligned(16))) __bf16 lds[4 * PER * FRAG];
+  const int tid = threadIdx.x;
+  const int64_t rgroups = p.RB / 4;  // 64-row groups (RB is a multiple of 4)
+  int64_t bid = blockIdx.x;
+  const int64_t rg = bid % rgroups;
+  bid /= rgroups;
+  const int64_t kb = bid % p.KB;
+  const int64_t h = bid / p.KB;
+  const S* __restrict__ X = (const S*)p.X + p.hv[h];
+  const int64_t r0 = rg * 64, k0 = kb * KBLK;
+  const bool conj = p.conj;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    int rl, kl;
+    if (p.kmajor) {
+      kl = tid & 31;
+      rl = (tid >> 5) + 8 * i;
+    } else {
+      rl = tid & 63;
+      kl = (tid >> 6) + 4 * i;
+    }
+    const int64_t r = r0 + rl, k = k0 + kl;
+    const bool ok = r < p.R && k < p.K;
+    const S v = X[ok ? p.rv[r] + p.kv[k] : 0];
+    const float re = ok ? St::re(v) : 0.0f;
+    const float im0 = ok ? St::im(v) : 0.0f;
+    const float im = conj ? -im0 : im0;
+    __bf16 t[TERMS];
+    const int pos = frag_pos(rl & 15, kl);
+    const int base = (rl >> 4) * PER * FRAG;
+    split<TERMS>(re, t);
+#pragma unroll
+    for (int x = 0; x < TERMS; x++) lds[base + x * FRAG + pos] = t[x];
+    if constexpr (CPLX) {
+      split<TERMS>(im, t);
+#pragma unroll
+      for (int x = 0; x < TERMS; x++) lds[base + (TERMS + x) * FRAG + pos] = t[x];
+    }
+  }
+  __syncthreads();
+  // The 4 row blocks are contiguous in the panel: copy 16 bytes per thread.
+  v4u* dst = (v4u*)(p.out + ((h * p.KB + kb) * p.RB + rg * 4) * PER * FRAG);
+  const v4u* src = (const v4u*)lds;
+  constexpr int CHUNKS = 4 * PER * FRAG * 2 / 16;
+#pragma unroll
+  for (int c = tid; c < CHUNKS; c += 256) dst[c] = src[c];
+}
+
+template <bool CPLX_, int TERMS_, int BM_, int BN_, int WM_, int WN_>
+struct GCfg {
+  static constexpr bool CPLX = CPLX_;
+  static constexpr int TERMS = TERMS_;
+  static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;
+  static constexpr int WAVES_M = BM / WM;
+  static constexpr int NT = (BM / WM) * (BN / WN) * 64;
+  static constexpr int COMPS = CPLX ? 2 : 1;
+  static constexpr int PER = COMPS * TERMS;
+  static constexpr int A_FRAGS = BM / 16 * PER, B_FRAGS = BN / 16 * PER;
+  static constexpr int CHUNKS = (A_FRAGS + B_FRAGS) * FRAG * 2 / 16;  // 16-byte chunks per K step
+  static constexpr int PT = CHUNKS / NT;                               // per thread
+  static constexpr int MI = WM / 16, NI = WN / 16;
+  static constexpr int LDD = BN + 1;
+  static constexpr size_t LDS_MAIN = (size_t)(A_FRAGS + B_FRAGS) * FRAG * 2;
+  static constexpr size_t LDS_EPI = (size_t)COMPS * BM * LDD * 4;
+  static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
+  static_assert(CHUNKS % NT == 0, "whole chunks per thread");
+};
+
+struct GemmArgs {
+  ContractArgs c;
+  const __bf16* Ap;
+  const __bf16* Bp;
+  int64_t RBa, RBb, KB;
+};
+
+template <class G>
+__global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
+  const ContractArgs& p = g.c;
+  __shared__ __attribute__((aligned(16))) char smem[G::LDS];
+  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
+  int64_t h;
+  int tm, tn;
+  dev::decode_tile(blockIdx.x, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
+  constexpr int BM = G::BM, BN = G::BN, NT = G::NT, PER = G::PER;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = (wave % C::WAVES_M) * C::WM;
-  const int wn = (wave / C::WAVES_M) * C::WN;
+  const int wm = (wave % G::WAVES_M) * G::WM;
+  const int wn = (wave / G::WAVES_M) * G::WN;
 
-  const S* __restrict__ A = (const S*)p.A + p.ha[h];
-  const S* __restrict__ B = (const S*)p.B + p.hb[h];
-  const int64_t M = p.M, N = p.N, K = p.K;
-  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-  const int64_t* __restrict__ ak = p.ak;
-  const int64_t* __restrict__ bk = p.bk;
+  __bf16* As = (__bf16*)smem;          // [BM/16][PER][FRAG]
+  __bf16* Bs = As + G::A_FRAGS * FRAG;  // [BN/16][PER][FRAG]
+  // Panel chunks of this tile for K block kb: A's BM/16 row blocks then B's.
+  const v4u* Ag = (const v4u*)(g.Ap + (h * g.KB * g.RBa + (int64_t)tm * (BM / 16)) * PER * FRAG);
+  const v4u* Bg = (const v4u*)(g.Bp + (h * g.KB * g.RBb + (int64_t)tn * (BN / 16)) * PER * FRAG);
+  const int64_t a_step = g.RBa * PER * FRAG * 2 / 16;  // chunks per K block
+  const int64_t b_step = g.RBb * PER * FRAG * 2 / 16;
+  constexpr int A_CH = G::A_FRAGS * FRAG * 2 / 16;
 
-  __bf16* As = (__bf16*)smem;                                // [COMPS][TERMS][BM][LDK]
-  __bf16* Bs = As + C::COMPS * C::TERMS * C::A_PLANE;        // [COMPS][TERMS][BN][LDK]
-
-  // Run coordinates: K-major: one k-group, RUNS rows; M-major: one row, RUNS k-groups.
-  const int a_g0 = C::AKM ? tid % KG : tid / BM;
-  const int a_m0 = C::AKM ? tid / KG : tid % BM;
-  constexpr int a_gstep = C::AKM ? 0 : NT / BM;
-  constexpr int a_mstep = C::AKM ? NT / KG : 0;
-  const int b_g0 = C::BKM ? tid % KG : tid / BN;
-  const int b_n0 = C::BKM ? tid / KG : tid % BN;
-  constexpr int b_gstep = C::BKM ? 0 : NT / BN;
-  constexpr int b_nstep = C::BKM ? NT / KG : 0;
-
-  int64_t a_row[C::RA], b_col[C::RB];
-  uint32_t a_rok = 0, b_cok = 0;
+  v4u reg[G::PT];
+  auto fetch = [&](int64_t kb) {
 #pragma unroll
-  for (int r = 0; r < C::RA; r++) {
-    const int64_t m = m0 + a_m0 + r * a_mstep;
-    const bool ok = m < M;
-    a_row[r] = p.am[ok ? m : 0];
-    a_rok |= (uint32_t)ok << r;
-  }
-#pragma unroll
-  for (int r = 0; r < C::RB; r++) {
-    const int64_t n = n0 + b_n0 + r * b_nstep;
-    const bool ok = n < N;
-    b_col[r] = p.bn[ok ? n : 0];
-    b_cok |= (uint32_t)ok << r;
-  }
-
-  int64_t a_ko[C::KA], b_ko[C::KB];
-  uint32_t a_kok = 0, b_kok = 0;
-  const int64_t kmax = K - 1;
-  auto load_koff = [&](int64_t k0) {
-    a_kok = 0;
-    b_kok = 0;
-#pragma unroll
-    for (int q = 0; q < C::KA; q++) {
-      const int g = a_g0 + (q / KV) * a_gstep;
-      const int64_t k = k0 + g * KV + q % KV;
-      a_ko[q] = ak[k < K ? k : kmax];
-      a_kok |= (uint32_t)(k < K) << q;
-    }
-#pragma unroll
-    for (int q = 0; q < C::KB; q++) {
-      const int g = b_g0 + (q / KV) * b_gstep;
-      const int64_t k = k0 + g * KV + q % KV;
-      b_ko[q] = bk[k < K ? k : kmax];
-      b_kok |= (uint32_t)(k < K) << q;
+    for (int i = 0; i < G::PT; i++) {
+      const int c = tid + i * NT;
+      reg[i] = c < A_CH ? Ag[kb * a_step + c] : Bg[kb * b_step + (c - A_CH)];
     }
   };
-
-  S ra[RUNS_A][KV], rb[RUNS_B][KV];
-  uint32_t a_live = 0, b_live = 0;
-  auto gather = [&]() {
-    a_live = 0;
-    b_live = 0;
-#pragma unroll
-    for (int r = 0; r < RUNS_A; r++)
-#pragma unroll
-      for (int v = 0; v < KV; v++) {
-        const int ri = C::AKM ? r : 0, q = C::AKM ? v : r * KV + v;
-        const bool ok = ((a_rok >> ri) & 1) && ((a_kok >> q) & 1);
-        ra[r][v] = A[ok ? a_row[ri] + a_ko[q] : 0];
-        a_live |= (uint32_t)ok << (r * KV + v);
-      }
-#pragma unroll
-    for (int r = 0; r < RUNS_B; r++)
-#pragma unroll
-      for (int v = 0; v < KV; v++) {
-        const int ri = C::BKM ? r : 0, q = C::BKM ? v : r * KV + v;
-        const bool ok = ((b_cok >> ri) & 1) && ((b_kok >> q) & 1);
-        rb[r][v] = B[ok ? b_col[ri] + b_ko[q] : 0];
-        b_live |= (uint32_t)ok << (r * KV + v);
-      }
-  };
-
-  const bool ca = p.conj_a, cb = p.conj_b;
   auto stash = [&]() {
+    v4u* dst = (v4u*)As;
 #pragma unroll
-    for (int r = 0; r < RUNS_A; r++) {
-      const int m = a_m0 + r * a_mstep, g = a_g0 + r * a_gstep;
-      v4bf t[C::COMPS][C::TERMS];
-#pragma unroll
-      for (int v = 0; v < KV; v++) {
-        const bool ok = (a_live >> (r * KV + v)) & 1;
-        __bf16 s[C::TERMS];
-        split<C::TERMS>(ok ? St::re(ra[r][v]) : 0.0f, s);
-#pragma unroll
-        for (int x = 0; x < C::TERMS; x++) t[0][x][v] = s[x];
-        if constexpr (C::CPLX) {
-          float im = ok ? St::im(ra[r][v]) : 0.0f;
-          split<C::TERMS>(ca ? -im : im, s);
-#pragma unroll
-          for (int x = 0; x < C::TERMS; x++) t[1][x][v] = s[x];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < C::COMPS; c++)
-#pragma unroll
-        for (int x = 0; x < C::TERMS; x++)
-          *(v4bf*)&As[(c * C::TERMS + x) * C::A_PLANE + m * C::LDK + g * KV] = t[c][x];
-    }
-#pragma unroll
-    for (int r = 0; r < RUNS_B; r++) {
-      const int n = b_n0 + r * b_nstep, g = b_g0 + r * b_gstep;
-      v4bf t[C::COMPS][C::TERMS];
-#pragma unroll
-      for (int v = 0; v < KV; v++) {
-        const bool ok = (b_live >> (r * KV + v)) & 1;
-        __bf16 s[C::TERMS];
-        split<C::TERMS>(ok ? St::re(rb[r][v]) : 0.0f, s);
-#pragma unroll
-        for (int x = 0; x < C::TERMS; x++) t[0][x][v] = s[x];
-        if constexpr (C::CPLX) {
-          float im = ok ? St::im(rb[r][v]) : 0.0f;
-          split<C::TERMS>(cb ? -im : im, s);
-#pragma unroll
-          for (int x = 0; x < C::TERMS; x++) t[1][x][v] = s[x];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < C::COMPS; c++)
-#pragma unroll
-        for (int x = 0; x < C::TERMS; x++)
-          *(v4bf*)&Bs[(c * C::TERMS + x) * C::B_PLANE + n * C::LDK + g * KV] = t[c][x];
-    }
+    for (int i = 0; i < G::PT; i++) dst[tid + i * NT] = reg[i];
   };
 
-  // Term pairs multiplied: (a term, b term).
-  constexpr int NP = C::TERMS == 2 ? 3 : 6;
+  constexpr int NP = G::TERMS == 2 ? 3 : 6;
   constexpr int PA[6] = {0, 0, 1, 1, 0, 2};
   constexpr int PB[6] = {0, 1, 0, 1, 2, 0};
-  // Accumulators: real [1]; complex [3] = rr, ii, ri + ir.
-  constexpr int ACCS = C::CPLX ? 3 : 1;
-  v4f acc[ACCS][C::MI][C::NI];
+  constexpr int ACCS = G::CPLX ? 3 : 1;
+  v4f acc[ACCS][G::MI][G::NI];
 #pragma unroll
   for (int s = 0; s < ACCS; s++)
 #pragma unroll
-    for (int i = 0; i < C::MI; i++)
+    for (int i = 0; i < G::MI; i++)
 #pragma unroll
-      for (int j = 0; j < C::NI; j++) acc[s][i][j] = v4f{0, 0, 0, 0};
+      for (int j = 0; j < G::NI; j++) acc[s][i][j] = v4f{0, 0, 0, 0};
 
-  const int lr = lane & 15, lk = lane >> 4;
-  const int64_t nk = (K + BK - 1) / BK;
-  load_koff(0);
-  gather();
-  if (nk > 1) load_koff(BK);
-  for (int64_t kt = 0; kt < nk; kt++) {
+  const int64_t nk = g.KB;
+  fetch(0);
+  for (int64_t kb = 0; kb < nk; kb++) {
     stash();
     __syncthreads();
-    if (kt + 1 < nk) {
-      gather();
-      if (kt + 2 < nk) load_koff((kt + 2) * BK);
-    }
+    if (kb + 1 < nk) fetch(kb + 1);
 #pragma unroll
-    for (int c = 0; c < C::COMPS; c++) {
+    for (int c = 0; c < G::COMPS; c++) {
 #pragma unroll
-      for (int d = 0; d < C::COMPS; d++) {
-        // (A component c) x (B component d) -> accumulator slot
-        const int slot = !C::CPLX ? 0 : (c == 0 && d == 0 ? 0 : (c == 1 && d == 1 ? 1 : 2));
-        v8bf a[C::TERMS][C::MI], b[C::TERMS][C::NI];
+      for (int d = 0; d < G::COMPS; d++) {
+        const int slot = !G::CPLX ? 0 : (c == 0 && d == 0 ? 0 : (c == 1 && d == 1 ? 1 : 2));
+        v8bf a[G::TERMS][G::MI], b[G::TERMS][G::NI];
 #pragma unroll
-        for (int x = 0; x < C::TERMS; x++) {
+        for (int x = 0; x < G::TERMS; x++) {
 #pragma unroll
-          for (int i = 0; i < C::MI; i++)
-            a[x][i] = *(const v8bf*)&As[(c * C::TERMS + x) * C::A_PLANE + (wm + 16 * i + lr) * C::LDK + 8 * lk];
+          for (int i = 0; i < G::MI; i++)
+            a[x][i] = *(const v8bf*)&As[(((wm >> 4) + i) * PER + c * G::TERMS + x) * FRAG + lane * 8];
 #pragma unroll
-          for (int j = 0; j < C::NI; j++)
-            b[x][j] = *(const v8bf*)&Bs[(d * C::TERMS + x) * C::B_PLANE + (wn + 16 * j + lr) * C::LDK + 8 * lk];
+          for (int j = 0; j < G::NI; j++)
+            b[x][j] = *(const v8bf*)&Bs[(((wn >> 4) + j) * PER + d * G::TERMS + x) * FRAG + lane * 8];
         }
 #pragma unroll
         for (int pp = 0; pp < NP; pp++)
 #pragma unroll
-          for (int i = 0; i < C::MI; i++)
+          for (int i = 0; i < G::MI; i++)
 #pragma unroll
-            for (int j = 0; j < C::NI; j++)
+            for (int j = 0; j < G::NI; j++)
               acc[slot][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[PA[pp]][i], b[PB[pp]][j], acc[slot][i][j],
                                                                         0, 0, 0);
       }
@@ -281,71 +241,90 @@ __device__ inline void split_tile(const ContractArgs& p, int64_t h, int tm, int 
   }
 
   float* Dre = (float*)smem;
-  float* Dim = Dre + BM * C::LDD;
+  float* Dim = Dre + BM * G::LDD;
+  const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int i = 0; i < C::MI; i++)
+  for (int i = 0; i < G::MI; i++)
 #pragma unroll
-    for (int j = 0; j < C::NI; j++)
+    for (int j = 0; j < G::NI; j++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int row = wm + 16 * i + 4 * lk + q, col = wn + 16 * j + lr;
-        if constexpr (C::CPLX) {
-          Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
-          Dim[row * C::LDD + col] = acc[2][i][j][q];
+        if constexpr (G::CPLX) {
+          Dre[row * G::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
+          Dim[row * G::LDD + col] = acc[2][i][j][q];
         } else {
-          Dre[row * C::LDD + col] = acc[0][i][j][q];
+          Dre[row * G::LDD + col] = acc[0][i][j][q];
         }
       }
   __syncthreads();
-  dev::write_back<float, float, C::CPLX, BM, BN, NT, C::LDD>(p, h, m0, n0, Dre, Dim);
+  dev::write_back<float, float, G::CPLX, BM, BN, NT, G::LDD>(p, h, (int64_t)tm * BM, (int64_t)tn * BN, Dre, Dim);
 }
 
-template <class C>
-__global__ __launch_bounds__(C::NT) void k_split(ContractArgs p) {
-  __shared__ __attribute__((aligned(16))) char smem[C::LDS];
-  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
-  int64_t h;
-  int tm, tn;
-  dev::decode_tile(blockIdx.x, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
-  split_tile<C>(p, h, tm, tn, smem);
-}
+struct Shape {
+  int bm, bn;
+};
 
-template <class C>
-hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
-  ContractArgs p = a;
-  p.tiles_m = (int)((p.M + C::BM - 1) / C::BM);
-  p.tiles_n = (int)((p.N + C::BN - 1) / C::BN);
-  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
-  if (nwg == 0) return hipSuccess;
-  if (nwg > 0x7fffffff) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(k_split<C>, dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
-  return hipGetLastError();
-}
-
-template <bool CPLX, int TERMS, int BM, int BN, int WM, int WN>
-hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
-  if (a.a_kmajor) {
-    if (a.b_kmajor) return launch_cfg<SCfg<CPLX, TERMS, BM, BN, WM, WN, true, true>>(a, s);
-    return launch_cfg<SCfg<CPLX, TERMS, BM, BN, WM, WN, true, false>>(a, s);
-  }
-  if (a.b_kmajor) return launch_cfg<SCfg<CPLX, TERMS, BM, BN, WM, WN, false, true>>(a, s);
-  return launch_cfg<SCfg<CPLX, TERMS, BM, BN, WM, WN, false, false>>(a, s);
+Shape shape_of(int tile_id, bool cplx) {
+  if (tile_id == STILE_128x128 && !cplx) return {128, 128};
+  return {64, 64};
 }
 
 template <bool CPLX, int TERMS>
-hipError_t launch_tile(const ContractArgs& a, int tile_id, hipStream_t s) {
-  if constexpr (!CPLX) {
-    if (tile_id == STILE_128x128) return launch_majors<CPLX, TERMS, 128, 128, 64, 64>(a, s);
+hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) {
+  constexpr int PER = (CPLX ? 2 : 1) * TERMS;
+  const Shape sh = shape_of(tile_id, CPLX);
+  const int64_t KB = (a.K + KBLK - 1) / KBLK;
+  const int64_t tiles_m = (a.M + sh.bm - 1) / sh.bm, tiles_n = (a.N + sh.bn - 1) / sh.bn;
+  // Row blocks per panel slice: whole tiles, and a multiple of 4 for the packer.
+  auto pad = [](int64_t rows, int bm) {
+    int64_t rb = (rows + bm - 1) / bm * (bm / 16);
+    return (rb + 3) / 4 * 4;
+  };
+  const int64_t RBa = pad(a.M, sh.bm), RBb = pad(a.N, sh.bn);
+  __bf16* Ap = (__bf16*)panels;
+  __bf16* Bp = Ap + a.H * KB * RBa * PER * FRAG;
+  PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kmajor, Ap};
+  PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kmajor, Bp};
+  for (const PackArgs* q : {&pa, &pb}) {
+    const int64_t grid = q->H * q->KB * (q->RB / 4);
+    if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_pack<CPLX, TERMS>), dim3((unsigned)grid), dim3(256), 0, s, *q);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  if (tile_id == STILE_64x64) return launch_majors<CPLX, TERMS, 64, 64, 32, 32>(a, s);
-  return hipErrorInvalidValue;
+  GemmArgs g{a, Ap, Bp, RBa, RBb, KB};
+  g.c.tiles_m = (int)tiles_m;
+  g.c.tiles_n = (int)tiles_n;
+  const int64_t nwg = tiles_m * tiles_n * a.H;
+  if (nwg > 0x7fffffff) return hipErrorInvalidConfiguration;
+  if (sh.bm == 128) {
+    if constexpr (!CPLX) hipLaunchKernelGGL((k_gemm<GCfg<CPLX, TERMS, 128, 128, 64, 64>>), dim3((unsigned)nwg),
+                                            dim3(256), 0, s, g);
+  } else {
+    hipLaunchKernelGGL((k_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32>>), dim3((unsigned)nwg), dim3(256), 0, s, g);
+  }
+  return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int tile_id, hipStream_t s) {
-  if (t == DType::F32) return nterms == 2 ? launch_tile<false, 2>(a, tile_id, s) : launch_tile<false, 3>(a, tile_id, s);
-  if (t == DType::C32) return nterms == 2 ? launch_tile<true, 2>(a, tile_id, s) : launch_tile<true, 3>(a, tile_id, s);
+size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id) {
+  const bool cplx = t == DType::C32;
+  const Shape sh = shape_of(tile_id, cplx);
+  const int64_t KB = (a.K + KBLK - 1) / KBLK;
+  auto pad = [](int64_t rows, int bm) {
+    int64_t rb = (rows + bm - 1) / bm * (bm / 16);
+    return (rb + 3) / 4 * 4;
+  };
+  const int64_t per = (cplx ? 2 : 1) * nterms;
+  return (size_t)a.H * KB * (pad(a.M, sh.bm) + pad(a.N, sh.bn)) * per * FRAG * 2;
+}
+
+hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int tile_id, void* panels,
+                                 hipStream_t s) {
+  if (t == DType::F32) return nterms == 2 ? run<false, 2>(a, tile_id, panels, s) : run<false, 3>(a, tile_id, panels, s);
+  if (t == DType::C32) return nterms == 2 ? run<true, 2>(a, tile_id, panels, s) : run<true, 3>(a, tile_id, panels, s);
   return hipErrorInvalidValue;
 }
 
