@@ -59,6 +59,13 @@ struct ZeroLayout {
   int64_t total;
 };
 
+// Zero n bytes starting at a 16-byte aligned address (dense layouts).
+__global__ void k_zero_bytes(uint4* base, int64_t n16, char* tail, int64_t ntail) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n16; e += stride) base[e] = uint4{0, 0, 0, 0};
+  if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
 template <typename S>
 __global__ void k_zero_layout(S* base, ZeroLayout z) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < z.total; e += (int64_t)gridDim.x * blockDim.x) {
@@ -92,15 +99,60 @@ hipError_t launch_output_pass(const ContractArgs& a, DType t, hipStream_t s) {
 hipError_t launch_zero_layout(void* base, DType t, int rank, const int64_t* dims, const int64_t* strides,
                               hipStream_t s) {
   if (rank > 8) return hipErrorInvalidValue;
-  ZeroLayout z;
-  z.rank = rank;
-  z.total = 1;
+  const int64_t es = elem_size(t);
+  // Normalise: positive strides (moving the origin), extent-1 axes dropped,
+  // sorted by stride, adjacent axes folded. The set of addressed elements is
+  // unchanged, and a dense layout becomes one axis of stride 1.
+  struct Ax {
+    int64_t e, s;
+  } ax[8];
+  int n = 0;
+  char* origin = (char*)base;
   for (int i = 0; i < rank; i++) {
-    z.dims[i] = dims[i];
-    z.strides[i] = strides[i];
-    z.total *= dims[i];
+    if (dims[i] == 0) return hipSuccess;
+    if (dims[i] == 1) continue;
+    int64_t s = strides[i];
+    if (s < 0) {
+      origin += (dims[i] - 1) * s * es;
+      s = -s;
+    }
+    ax[n++] = {dims[i], s};
   }
-  if (z.total == 0) return hipSuccess;
+  for (int i = 1; i < n; i++)
+    for (int j = i; j > 0 && ax[j].s < ax[j - 1].s; j--) {
+      Ax tmp = ax[j];
+      ax[j] = ax[j - 1];
+      ax[j - 1] = tmp;
+    }
+  int m = 0;
+  for (int i = 0; i < n; i++) {
+    if (m > 0 && ax[i].s == ax[m - 1].s * ax[m - 1].e) ax[m - 1].e *= ax[i].e;
+    else ax[m++] = ax[i];
+  }
+  if (m == 0 || (m == 1 && ax[0].s == 1)) {
+    const int64_t bytes = (m == 0 ? 1 : ax[0].e) * es;
+    const uintptr_t addr = (uintptr_t)origin;
+    const int64_t head = addr % 16 ? (int64_t)(16 - addr % 16) : 0;
+    if (head > 0 || bytes < 64) {
+      // Unaligned start or tiny: fall through to the element loop below.
+    } else {
+      const int64_t n16 = bytes / 16, ntail = bytes - n16 * 16;
+      int64_t blocks = (n16 + 255) / 256;
+      if (blocks > 65536) blocks = 65536;
+      if (blocks < 1) blocks = 1;
+      hipLaunchKernelGGL(k_zero_bytes, dim3(blocks), dim3(256), 0, s, (uint4*)origin, n16, origin + n16 * 16, ntail);
+      return hipGetLastError();
+    }
+  }
+  ZeroLayout z;
+  z.rank = m;
+  z.total = 1;
+  for (int i = 0; i < m; i++) {
+    z.dims[i] = ax[i].e;
+    z.strides[i] = ax[i].s;
+    z.total *= ax[i].e;
+  }
+  base = origin;
   const int threads = 256;
   int64_t blocks = (z.total + threads - 1) / threads;
   if (blocks > 65536) blocks = 65536;

@@ -485,6 +485,7 @@ const TileCfg* tile_cfg(int id) {
 #undef TP_ROW
       {STILE_128x128, 128, 128, 32, 256, "STILE_128x128"},
       {STILE_64x64, 64, 64, 32, 256, "STILE_64x64"},
+      {STILE_256x128, 256, 128, 32, 512, "STILE_256x128"},
   };
   for (auto& t : tiles)
     if (t.id == id) return &t;

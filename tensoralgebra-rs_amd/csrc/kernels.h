@@ -23,8 +23,9 @@ enum TileId : int {
 };
 // Tile ids of the split-BF16 kernels (16x16x32 bf16 MFMA family).
 enum SplitTileId : int {
-  STILE_128x128 = 11,  // 4 waves of 64x64
+  STILE_128x128 = 11,  // 4 waves of 64x64 (f32)
   STILE_64x64 = 12,    // 4 waves of 32x32
+  STILE_256x128 = 13,  // 8 waves of 64x64 (f32)
 };
 
 const TileCfg* tile_cfg(int id);

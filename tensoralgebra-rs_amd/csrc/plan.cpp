@@ -112,7 +112,8 @@ Axis scaled(const Axis& a, int64_t extent, int64_t by) {
 }
 
 bool tile_allowed(int id, DType t, int prec) {
-  if (prec != PREC_STORAGE) return id == STILE_64x64 || (id == STILE_128x128 && !is_complex(t));
+  if (prec != PREC_STORAGE)
+    return id == STILE_64x64 || ((id == STILE_128x128 || id == STILE_256x128) && !is_complex(t));
   if (id >= 10) return false;
   if (id == TILE_128x64 && t == DType::C64) return false;  // staging would exceed 64 KiB
   return tile_cfg(id) != nullptr;
@@ -126,7 +127,7 @@ int choose_tile(const Plan& p) {
   }
   std::vector<std::pair<int, double>> cands;
   if (prec != PREC_STORAGE) {
-    cands = {{STILE_128x128, 1.15}, {STILE_64x64, 1.0}};
+    cands = {{STILE_256x128, 1.3}, {STILE_128x128, 1.15}, {STILE_64x64, 1.0}};
   } else {
     cands = {{TILE_128x64, 1.1}, {TILE_64x64, 1.0}, {TILE_32x32, 0.7}};
   }

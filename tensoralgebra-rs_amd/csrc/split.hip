@@ -129,25 +129,35 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
   for (int c = tid; c < CHUNKS; c += 256) dst[c] = src[c];
 }
 
-template <bool CPLX_, int TERMS_, int BM_, int BN_, int WM_, int WN_>
+template <bool CPLX_, int TERMS_, int BM_, int BN_, int WM_, int WN_, int STAGES_>
 struct GCfg {
   static constexpr bool CPLX = CPLX_;
-  static constexpr int TERMS = TERMS_;
+  static constexpr int TERMS = TERMS_, STAGES = STAGES_;
   static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;
   static constexpr int WAVES_M = BM / WM;
-  static constexpr int NT = (BM / WM) * (BN / WN) * 64;
+  static constexpr int NW = (BM / WM) * (BN / WN);
+  static constexpr int NT = NW * 64;
   static constexpr int COMPS = CPLX ? 2 : 1;
   static constexpr int PER = COMPS * TERMS;
   static constexpr int A_FRAGS = BM / 16 * PER, B_FRAGS = BN / 16 * PER;
-  static constexpr int CHUNKS = (A_FRAGS + B_FRAGS) * FRAG * 2 / 16;  // 16-byte chunks per K step
-  static constexpr int PT = CHUNKS / NT;                               // per thread
+  static constexpr int FRAGS = A_FRAGS + B_FRAGS;  // 1 KiB fragments per K step
+  static constexpr int PW = FRAGS / NW;            // fragments each wave copies per K step
   static constexpr int MI = WM / 16, NI = WN / 16;
   static constexpr int LDD = BN + 1;
-  static constexpr size_t LDS_MAIN = (size_t)(A_FRAGS + B_FRAGS) * FRAG * 2;
+  static constexpr size_t STAGE_BYTES = (size_t)FRAGS * FRAG * 2;
+  static constexpr size_t LDS_MAIN = STAGES * STAGE_BYTES;
   static constexpr size_t LDS_EPI = (size_t)COMPS * BM * LDD * 4;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
-  static_assert(CHUNKS % NT == 0, "whole chunks per thread");
+  static_assert(FRAGS % NW == 0, "whole fragments per wave");
+  static_assert(LDS <= 160 * 1024, "LDS budget");
 };
+
+// s_waitcnt with only the vector-memory counter constrained (gfx9 encoding).
+template <int N>
+__device__ inline void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
 
 struct GemmArgs {
   ContractArgs c;
@@ -171,27 +181,26 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
   const int wm = (wave % G::WAVES_M) * G::WM;
   const int wn = (wave / G::WAVES_M) * G::WN;
 
-  __bf16* As = (__bf16*)smem;          // [BM/16][PER][FRAG]
-  __bf16* Bs = As + G::A_FRAGS * FRAG;  // [BN/16][PER][FRAG]
-  // Panel chunks of this tile for K block kb: A's BM/16 row blocks then B's.
-  const v4u* Ag = (const v4u*)(g.Ap + (h * g.KB * g.RBa + (int64_t)tm * (BM / 16)) * PER * FRAG);
-  const v4u* Bg = (const v4u*)(g.Bp + (h * g.KB * g.RBb + (int64_t)tn * (BN / 16)) * PER * FRAG);
-  const int64_t a_step = g.RBa * PER * FRAG * 2 / 16;  // chunks per K block
-  const int64_t b_step = g.RBb * PER * FRAG * 2 / 16;
-  constexpr int A_CH = G::A_FRAGS * FRAG * 2 / 16;
-
-  v4u reg[G::PT];
-  auto fetch = [&](int64_t kb) {
+  // Panel fragments of this tile for K block kb: A's BM/16 row blocks then B's,
+  // each group contiguous in its panel. A stage in LDS holds them in the same
+  // order: [A_FRAGS][FRAG] then [B_FRAGS][FRAG].
+  const __bf16* Ag = g.Ap + (h * g.KB * g.RBa + (int64_t)tm * (BM / 16)) * PER * FRAG;
+  const __bf16* Bg = g.Bp + (h * g.KB * g.RBb + (int64_t)tn * (BN / 16)) * PER * FRAG;
+  const int64_t a_step = g.RBa * PER * FRAG;  // bf16 per K block
+  const int64_t b_step = g.RBb * PER * FRAG;
+  // Copies K block kb into LDS stage `st`, straight from global memory: one
+  // instruction per fragment, lane l moving bytes [16 l, 16 l + 16).
+  auto issue = [&](int64_t kb, int st) {
+    char* stage = smem + (size_t)st * G::STAGE_BYTES;
 #pragma unroll
-    for (int i = 0; i < G::PT; i++) {
-      const int c = tid + i * NT;
-      reg[i] = c < A_CH ? Ag[kb * a_step + c] : Bg[kb * b_step + (c - A_CH)];
+    for (int i = 0; i < G::PW; i++) {
+      const int f = wave + i * G::NW;  // wave-uniform fragment index
+      const __bf16* src = f < G::A_FRAGS ? Ag + kb * a_step + (int64_t)f * FRAG
+                                         : Bg + kb * b_step + (int64_t)(f - G::A_FRAGS) * FRAG;
+      __builtin_amdgcn_global_load_lds((const void*)(src + lane * 8),
+                                       (__attribute__((address_space(3))) void*)(stage + (size_t)f * FRAG * 2), 16,
+                                       0, 0);
     }
-  };
-  auto stash = [&]() {
-    v4u* dst = (v4u*)As;
-#pragma unroll
-    for (int i = 0; i < G::PT; i++) dst[tid + i * NT] = reg[i];
   };
 
   constexpr int NP = G::TERMS == 2 ? 3 : 6;
@@ -206,12 +215,25 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
 #pragma unroll
       for (int j = 0; j < G::NI; j++) acc[s][i][j] = v4f{0, 0, 0, 0};
 
+  // Pipeline: K block kb lives in stage kb % STAGES. Before computing block kb
+  // a wave waits for its own copies of kb (vmcnt), the barrier publishes
+  // everyone's and guarantees every wave has finished block kb - 1, whose
+  // stage then receives block kb + STAGES - 1.
   const int64_t nk = g.KB;
-  fetch(0);
+#pragma unroll
+  for (int s = 0; s < G::STAGES - 1; s++)
+    if (s < nk) issue(s, s);
   for (int64_t kb = 0; kb < nk; kb++) {
-    stash();
+    if constexpr (G::STAGES == 3) {
+      if (kb + 1 < nk) wait_vmcnt<G::PW>();
+      else wait_vmcnt<0>();
+    } else {
+      wait_vmcnt<0>();
+    }
     __syncthreads();
-    if (kb + 1 < nk) fetch(kb + 1);
+    if (kb + G::STAGES - 1 < nk) issue(kb + G::STAGES - 1, (int)((kb + G::STAGES - 1) % G::STAGES));
+    const __bf16* As = (const __bf16*)(smem + (size_t)(kb % G::STAGES) * G::STAGE_BYTES);
+    const __bf16* Bs = As + G::A_FRAGS * FRAG;
 #pragma unroll
     for (int c = 0; c < G::COMPS; c++) {
 #pragma unroll
@@ -237,8 +259,8 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
                                                                         0, 0, 0);
       }
     }
-    __syncthreads();
   }
+  __syncthreads();  // every wave is done with the stages before they become the staging tile
 
   float* Dre = (float*)smem;
   float* Dim = Dre + BM * G::LDD;
@@ -266,8 +288,14 @@ struct Shape {
 };
 
 Shape shape_of(int tile_id, bool cplx) {
+  if (tile_id == STILE_256x128 && !cplx) return {256, 128};
   if (tile_id == STILE_128x128 && !cplx) return {128, 128};
   return {64, 64};
+}
+
+template <class G>
+void launch_gemm(const GemmArgs& g, int64_t nwg, hipStream_t s) {
+  hipLaunchKernelGGL(k_gemm<G>, dim3((unsigned)nwg), dim3(G::NT), 0, s, g);
 }
 
 template <bool CPLX, int TERMS>
@@ -298,11 +326,12 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
   g.c.tiles_n = (int)tiles_n;
   const int64_t nwg = tiles_m * tiles_n * a.H;
   if (nwg > 0x7fffffff) return hipErrorInvalidConfiguration;
-  if (sh.bm == 128) {
-    if constexpr (!CPLX) hipLaunchKernelGGL((k_gemm<GCfg<CPLX, TERMS, 128, 128, 64, 64>>), dim3((unsigned)nwg),
-                                            dim3(256), 0, s, g);
+  if constexpr (!CPLX) {
+    if (sh.bm == 256) launch_gemm<GCfg<CPLX, TERMS, 256, 128, 64, 64, 2>>(g, nwg, s);
+    else if (sh.bm == 128) launch_gemm<GCfg<CPLX, TERMS, 128, 128, 64, 64, 2>>(g, nwg, s);
+    else launch_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32, 2>>(g, nwg, s);
   } else {
-    hipLaunchKernelGGL((k_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32>>), dim3((unsigned)nwg), dim3(256), 0, s, g);
+    launch_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32, 2>>(g, nwg, s);
   }
   return hipGetLastError();
 }
