@@ -18,14 +18,18 @@
 // otherwise), which keeps both the stores and the MFMA fragment reads free of
 // bank conflicts. Waves run 16x16x4 MFMAs (f64 or f32) from the planes.
 //
-// Software pipeline per K step: store the slab fetched last step, barrier,
-// issue the gathers of the next slab (their K offsets were loaded one step
-// earlier, so no load waits on another) and the offset loads of the slab
-// after it, then the MFMAs of the current slab, barrier. Out-of-range lanes
+// Software pipeline per K step over two LDS stages: the MFMAs of the current
+// slab, then the store of the next slab (fetched during the previous step)
+// into the other stage, the gathers of the slab after it (their K offsets
+// were loaded one step earlier, so no load waits on another) and the offset
+// loads of the one after that, then a single barrier. The stores and gathers
+// do not touch the stage being multiplied, so they overlap the MFMAs. Out-of-range lanes
 // load the operand's first element and discard it: the gathers have no
 // branches. The finished tile is staged through LDS and written back along
 // D's contiguous axis with alpha, beta, op_C and op_D fused.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "device_common.h"
 #include "kernels.h"
@@ -63,9 +67,10 @@ struct Mma<float> {
 
 enum { M_REAL = -1, M_NATIVE = 0, M_1M = 1, M_3M = 2 };
 
-template <typename T, int METHOD, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_>
+template <typename T, int METHOD, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_, bool BUF_>
 struct Cfg {
   typedef T Tv;
+  static constexpr bool BUF = BUF_;  // 32-bit buffer offsets (ContractArgs::off32)
   static constexpr int METH = METHOD;
   static constexpr bool CPLX = METHOD != M_REAL;
   static constexpr bool AKM = AKM_, BKM = BKM_;
@@ -73,12 +78,15 @@ struct Cfg {
   static constexpr int WAVES_M = BM / WM;
   static constexpr int NT = (BM / WM) * (BN / WN) * 64;
   static constexpr int PLANES = METHOD == M_REAL ? 1 : (METHOD == M_3M ? 3 : 2);
-  // Plane layouts: K-major gathers store [m][k] with an odd pitch BK+1 (16
-  // consecutive rows then fall on distinct banks); M-major gathers store
-  // [k][m] with pitch BM+16. Both keep the 16-row x 4-k fragment reads and
-  // the gather stores free of bank conflicts.
-  static constexpr int LDAK = BK + 1, LDAM = BM + 16;
-  static constexpr int LDBK = BK + 1, LDBN = BN + 16;
+  // Plane layouts: K-major gathers store [m][k], M-major gathers [k][m] with
+  // pitch BM+16. The K-major pitch puts the 16 rows of a fragment read on
+  // distinct banks of the 64 x 4-byte LDS banks: an odd pitch (BK+1 words)
+  // for f32; for f64 BK+2 doubles (a multiple of 4 words that is an odd
+  // multiple of 4 modulo 64), so that a half-wave's 32 eight-byte reads (16
+  // rows x 2 k) cover all 64 banks exactly once.
+  static constexpr int KPAD = sizeof(T) == 8 ? 2 : 1;
+  static constexpr int LDAK = BK + KPAD, LDAM = BM + 16;
+  static constexpr int LDBK = BK + KPAD, LDBN = BN + 16;
   static constexpr int A_PLANE = AKM ? BM * LDAK : BK * LDAM;
   static constexpr int B_PLANE = BKM ? BN * LDBK : BK * LDBN;
   static constexpr int LDD = BN + 1;  // row pitch of the staging tile
@@ -90,7 +98,8 @@ struct Cfg {
   static constexpr int RB = BKM ? EB : 1;
   static constexpr int MI = WM / 16, NI = WN / 16;
   static constexpr int ACCS = METHOD == M_REAL ? 1 : (METHOD == M_3M ? 3 : 2);
-  static constexpr size_t LDS_MAIN = (size_t)PLANES * (A_PLANE + B_PLANE) * sizeof(T);
+  static constexpr int STAGE = PLANES * (A_PLANE + B_PLANE);  // elements of one LDS stage
+  static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * sizeof(T);
   static constexpr size_t LDS_EPI = (size_t)(CPLX ? 2 : 1) * BM * LDD * sizeof(T);
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
   __device__ static inline int aidx(int m, int k) { return AKM ? m * LDAK + k : k * LDAM + m; }
@@ -124,9 +133,16 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   const int64_t* __restrict__ ak = p.ak;
   const int64_t* __restrict__ bk = p.bk;
+  // Buffer resources over the batch entry (used by the BUF variant only).
+  const __amdgpu_buffer_rsrc_t rsA = dev::make_rsrc(A);
+  const __amdgpu_buffer_rsrc_t rsB = dev::make_rsrc(B);
+  constexpr int64_t ES = sizeof(S);
+  // Row and K offsets: elements (64-bit) or, under BUF, bytes (32-bit) with
+  // out-of-range entries set to the sentinels.
+  typedef typename std::conditional<C::BUF, uint32_t, int64_t>::type Off;
 
-  T* As = (T*)smem;                     // [PLANES][A_PLANE]
-  T* Bs = As + C::PLANES * C::A_PLANE;  // [PLANES][B_PLANE]
+  // Two LDS stages, each [PLANES][A_PLANE] then [PLANES][B_PLANE].
+  T* const lds = (T*)smem;
 
   // Gather coordinates. K-major: one k per thread, EA rows. M-major: one row,
   // EA k's. (Likewise for B.)
@@ -139,41 +155,49 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
   constexpr int b_kstep = C::BKM ? 0 : NT / BN;
   constexpr int b_nstep = C::BKM ? NT / BK : 0;
 
-  int64_t a_row[C::RA], b_col[C::RB];
+  Off a_row[C::RA], b_col[C::RB];
   uint32_t a_rok = 0, b_cok = 0;
 #pragma unroll
   for (int r = 0; r < C::RA; r++) {
     const int64_t m = m0 + a_m0 + r * a_mstep;
     const bool ok = m < M;
-    a_row[r] = p.am[ok ? m : 0];
+    const int64_t v = p.am[ok ? m : 0];
+    if constexpr (C::BUF) a_row[r] = ok ? (uint32_t)(v * ES) : dev::ROW_OOB;
+    else a_row[r] = v;
     a_rok |= (uint32_t)ok << r;
   }
 #pragma unroll
   for (int r = 0; r < C::RB; r++) {
     const int64_t n = n0 + b_n0 + r * b_nstep;
     const bool ok = n < N;
-    b_col[r] = p.bn[ok ? n : 0];
+    const int64_t v = p.bn[ok ? n : 0];
+    if constexpr (C::BUF) b_col[r] = ok ? (uint32_t)(v * ES) : dev::ROW_OOB;
+    else b_col[r] = v;
     b_cok |= (uint32_t)ok << r;
   }
 
   // K offsets of the slab to gather next (loaded one step ahead).
-  int64_t a_ko[C::KA], b_ko[C::KB];
+  Off a_ko[C::KA], b_ko[C::KB];
   uint32_t a_kok = 0, b_kok = 0;
   const int64_t kmax = K - 1;
-#define TP_LOAD_KOFF(k0)                                          \
-  {                                                               \
-    a_kok = 0;                                                    \
-    b_kok = 0;                                                    \
-    _Pragma("unroll") for (int q = 0; q < C::KA; q++) {           \
-      const int64_t k = (k0) + a_k0 + q * a_kstep;                \
-      a_ko[q] = ak[k < K ? k : kmax];                             \
-      a_kok |= (uint32_t)(k < K) << q;                            \
-    }                                                             \
-    _Pragma("unroll") for (int q = 0; q < C::KB; q++) {           \
-      const int64_t k = (k0) + b_k0 + q * b_kstep;                \
-      b_ko[q] = bk[k < K ? k : kmax];                             \
-      b_kok |= (uint32_t)(k < K) << q;                            \
-    }                                                             \
+#define TP_LOAD_KOFF(k0)                                                       \
+  {                                                                            \
+    a_kok = 0;                                                                 \
+    b_kok = 0;                                                                 \
+    _Pragma("unroll") for (int q = 0; q < C::KA; q++) {                        \
+      const int64_t k = (k0) + a_k0 + q * a_kstep;                             \
+      const int64_t v = ak[k < K ? k : kmax];                                  \
+      if constexpr (C::BUF) a_ko[q] = k < K ? (uint32_t)(v * ES) : dev::K_OOB; \
+      else a_ko[q] = v;                                                        \
+      a_kok |= (uint32_t)(k < K) << q;                                         \
+    }                                                                          \
+    _Pragma("unroll") for (int q = 0; q < C::KB; q++) {                        \
+      const int64_t k = (k0) + b_k0 + q * b_kstep;                             \
+      const int64_t v = bk[k < K ? k : kmax];                                  \
+      if constexpr (C::BUF) b_ko[q] = k < K ? (uint32_t)(v * ES) : dev::K_OOB; \
+      else b_ko[q] = v;                                                        \
+      b_kok |= (uint32_t)(k < K) << q;                                         \
+    }                                                                          \
   }
 
   S ra[EA], rb[EB];
@@ -185,22 +209,26 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
     _Pragma("unroll") for (int i = 0; i < EA; i++) {                             \
       const int r = C::AKM ? i : 0, q = C::AKM ? 0 : i;                          \
       const bool ok = ((a_rok >> r) & 1) && ((a_kok >> q) & 1);                 \
-      ra[i] = A[ok ? a_row[r] + a_ko[q] : 0];                                    \
+      if constexpr (C::BUF) ra[i] = dev::buf_load<S>(rsA, a_row[r] + a_ko[q]);  \
+      else ra[i] = A[ok ? a_row[r] + a_ko[q] : 0];                               \
       a_live |= (uint32_t)ok << i;                                               \
     }                                                                            \
     _Pragma("unroll") for (int i = 0; i < EB; i++) {                             \
       const int r = C::BKM ? i : 0, q = C::BKM ? 0 : i;                          \
       const bool ok = ((b_cok >> r) & 1) && ((b_kok >> q) & 1);                  \
-      rb[i] = B[ok ? b_col[r] + b_ko[q] : 0];                                    \
+      if constexpr (C::BUF) rb[i] = dev::buf_load<S>(rsB, b_col[r] + b_ko[q]);  \
+      else rb[i] = B[ok ? b_col[r] + b_ko[q] : 0];                               \
       b_live |= (uint32_t)ok << i;                                               \
     }                                                                            \
   }
 
   const bool ca = p.conj_a, cb = p.conj_b;
-  auto stash = [&]() {
+  auto stash = [&](int buf) {
+    T* As = lds + buf * C::STAGE;
+    T* Bs = As + C::PLANES * C::A_PLANE;
 #pragma unroll
     for (int i = 0; i < EA; i++) {
-      const bool ok = (a_live >> i) & 1;
+      const bool ok = C::BUF || ((a_live >> i) & 1);  // BUF: out-of-range loads read zero
       T* dst = As + C::aidx(a_m0 + i * a_mstep, a_k0 + i * a_kstep);
       const T re = ok ? St::re(ra[i]) : T(0);
       dst[0] = re;
@@ -213,7 +241,7 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
     }
 #pragma unroll
     for (int i = 0; i < EB; i++) {
-      const bool ok = (b_live >> i) & 1;
+      const bool ok = C::BUF || ((b_live >> i) & 1);
       T* dst = Bs + C::bidx(b_n0 + i * b_nstep, b_k0 + i * b_kstep);
       const T re = ok ? St::re(rb[i]) : T(0);
       dst[0] = re;
@@ -241,13 +269,19 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
   TP_LOAD_KOFF(0);
   TP_GATHER();
   if (nk > 1) TP_LOAD_KOFF(BK);
+  stash(0);
+  if (nk > 1) {
+    TP_GATHER();
+    if (nk > 2) TP_LOAD_KOFF(2 * BK);
+  }
+  __syncthreads();
   for (int64_t kt = 0; kt < nk; kt++) {
-    stash();
-    __syncthreads();
-    if (kt + 1 < nk) {
-      TP_GATHER();
-      if (kt + 2 < nk) TP_LOAD_KOFF((kt + 2) * BK);
-    }
+    // Slab kt is in stage kt & 1. Its MFMAs run while slab kt + 1 (gathered
+    // during the previous step) is stored into the other stage and slab kt + 2
+    // is gathered; one barrier per step then publishes slab kt + 1 and retires
+    // stage kt & 1 for slab kt + 2.
+    const T* As = lds + (kt & 1) * C::STAGE;
+    const T* Bs = As + C::PLANES * C::A_PLANE;
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       if constexpr (C::METH == M_REAL) {
@@ -328,6 +362,13 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
         }
       }
     }
+    // Unconditional (branch-free, so the compiler can interleave them with
+    // the MFMAs): past the last slab the store rewrites the idle stage with
+    // data nobody reads, and the offsets and gathers fall back to every
+    // lane's safe address (K offsets clamp to K - 1, gathers to element 0).
+    stash((int)((kt + 1) & 1));
+    TP_GATHER();
+    TP_LOAD_KOFF((kt + 3) * BK);
     __syncthreads();
   }
 #undef TP_LOAD_KOFF
@@ -377,16 +418,38 @@ __global__ __launch_bounds__(C::NT) void k_contract(ContractArgs p) {
   contract_tile<C>(p, h, tm, tn, smem);
 }
 
+template <typename T, int METH, int BM, int BN, int BK, int WM, int WN, bool BUF>
+struct Majors {
+  typedef Cfg<T, METH, BM, BN, BK, WM, WN, true, true, BUF> KK;
+  typedef Cfg<T, METH, BM, BN, BK, WM, WN, true, false, BUF> KN;
+  typedef Cfg<T, METH, BM, BN, BK, WM, WN, false, true, BUF> MK;
+  typedef Cfg<T, METH, BM, BN, BK, WM, WN, false, false, BUF> MN;
+};
+
+// The eight gather variants of one tile: operand majors x offset width.
 template <typename T, int METH, int BM, int BN, int BK, int WM, int WN>
 struct Family {
-  typedef Cfg<T, METH, BM, BN, BK, WM, WN, true, true> KK;
-  typedef Cfg<T, METH, BM, BN, BK, WM, WN, true, false> KN;
-  typedef Cfg<T, METH, BM, BN, BK, WM, WN, false, true> MK;
-  typedef Cfg<T, METH, BM, BN, BK, WM, WN, false, false> MN;
+  typedef Majors<T, METH, BM, BN, BK, WM, WN, false> W64;
+  typedef Majors<T, METH, BM, BN, BK, WM, WN, true> W32;
+  typedef typename W64::KK KK;
+  typedef typename W64::KN KN;
+  typedef typename W64::MK MK;
+  typedef typename W64::MN MN;
   static constexpr int NT = KK::NT;
   static constexpr size_t LDS = KK::LDS > MN::LDS ? (KK::LDS > KN::LDS ? KK::LDS : KN::LDS)
                                                   : (MN::LDS > MK::LDS ? MN::LDS : MK::LDS);
 };
+
+template <class W>
+__device__ inline void tile_by_major(const ContractArgs& p, int64_t h, int tm, int tn, char* smem) {
+  if (p.a_kmajor) {
+    if (p.b_kmajor) contract_tile<typename W::KK>(p, h, tm, tn, smem);
+    else contract_tile<typename W::KN>(p, h, tm, tn, smem);
+  } else {
+    if (p.b_kmajor) contract_tile<typename W::MK>(p, h, tm, tn, smem);
+    else contract_tile<typename W::MN>(p, h, tm, tn, smem);
+  }
+}
 
 // Persistent grouped kernel: workgroups stride over the flattened tile space of
 // every item; each tile finds its item by binary search over tile_begin and
@@ -408,13 +471,8 @@ __global__ __launch_bounds__(F::NT) void k_grouped(const GroupItem* __restrict__
     const int64_t h = local / per_h;
     const int r = (int)(local - h * per_h);
     const int tm = r % p.tiles_m, tn = r / p.tiles_m;
-    if (p.a_kmajor) {
-      if (p.b_kmajor) contract_tile<typename F::KK>(p, h, tm, tn, smem);
-      else contract_tile<typename F::KN>(p, h, tm, tn, smem);
-    } else {
-      if (p.b_kmajor) contract_tile<typename F::MK>(p, h, tm, tn, smem);
-      else contract_tile<typename F::MN>(p, h, tm, tn, smem);
-    }
+    if (p.off32) tile_by_major<typename F::W32>(p, h, tm, tn, smem);
+    else tile_by_major<typename F::W64>(p, h, tm, tn, smem);
     __syncthreads();
   }
 }
@@ -431,21 +489,34 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <class W>
+hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
+  if (a.a_kmajor) {
+    if (a.b_kmajor) return launch_cfg<typename W::KK>(a, s);
+    return launch_cfg<typename W::KN>(a, s);
+  }
+  if (a.b_kmajor) return launch_cfg<typename W::MK>(a, s);
+  return launch_cfg<typename W::MN>(a, s);
+}
+
 template <class F>
 hipError_t launch_family(const ContractArgs& a, hipStream_t s) {
-  if (a.a_kmajor) {
-    if (a.b_kmajor) return launch_cfg<typename F::KK>(a, s);
-    return launch_cfg<typename F::KN>(a, s);
+  if constexpr (F::LDS > 160 * 1024) {
+    return hipErrorInvalidValue;  // shape/method combination the planner never picks
+  } else {
+    return a.off32 ? launch_majors<typename F::W32>(a, s) : launch_majors<typename F::W64>(a, s);
   }
-  if (a.b_kmajor) return launch_cfg<typename F::MK>(a, s);
-  return launch_cfg<typename F::MN>(a, s);
 }
 
 template <class F>
 hipError_t launch_group(const GroupItem* items, int num_items, int64_t total, int grid, hipStream_t s) {
-  if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_grouped<F>, dim3((unsigned)grid), dim3(F::NT), 0, s, items, num_items, total);
-  return hipGetLastError();
+  if constexpr (F::LDS > 160 * 1024) {
+    return hipErrorInvalidValue;  // shape/method combination the planner never picks
+  } else {
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grouped<F>, dim3((unsigned)grid), dim3(F::NT), 0, s, items, num_items, total);
+    return hipGetLastError();
+  }
 }
 
 // Tile shapes (BM, BN, BK, WM, WN) per tile id.
@@ -486,6 +557,7 @@ const TileCfg* tile_cfg(int id) {
       {STILE_128x128, 128, 128, 32, 256, "STILE_128x128"},
       {STILE_64x64, 64, 64, 32, 256, "STILE_64x64"},
       {STILE_256x128, 256, 128, 32, 512, "STILE_256x128"},
+      {STILE_128x64, 128, 64, 32, 512, "STILE_128x64"},
   };
   for (auto& t : tiles)
     if (t.id == id) return &t;

@@ -14,6 +14,28 @@ struct alignas(2 * sizeof(T)) Cx {
   T re, im;
 };
 
+// Raw buffer loads: 32-bit byte offsets from a base through a buffer resource
+// of BUF_RANGE bytes. An offset at or past the range reads zero, so gathers
+// mark an out-of-range row or k with a sentinel instead of selecting: every
+// valid offset is below BUF_RANGE (checked on the host), and any sum holding
+// one or both sentinels lands in [BUF_RANGE, 2^32).
+constexpr uint32_t BUF_RANGE = 1u << 30;
+constexpr uint32_t ROW_OOB = 0x80000000u;
+constexpr uint32_t K_OOB = 0x40000000u;
+// Word 3 of a gfx9 buffer resource for untyped dword access.
+constexpr int BUF_FLAGS = 0x00020000;
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)BUF_RANGE, BUF_FLAGS);
+}
+
+template <typename S>
+__device__ inline S buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(S) == 4) return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+  else if constexpr (sizeof(S) == 8) return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+  else return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+
 // Storage element of a dtype.
 template <typename T, bool CPLX>
 struct Store;

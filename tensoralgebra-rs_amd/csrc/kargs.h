@@ -38,7 +38,15 @@ struct ContractArgs {
   int32_t d_mmajor;  // the write-back runs along M (1) or along N (0)
   int32_t read_c;    // beta != 0: C is read (never read otherwise)
   int32_t tiles_m, tiles_n;
-  int32_t pad0, pad1;
+  // The K scatter of A (B) is the identity, K % 8 == 0 and every row and batch
+  // offset of the operand is 16-byte aligned: 8 consecutive k of one row load
+  // as whole 16-byte vectors (used by the split-BF16 packer).
+  int32_t a_kvec, b_kvec;
+  // Every gather offset of A and of B (rows plus k, in bytes from the batch
+  // entry's base) is nonnegative and below 2^30: gathers use 32-bit buffer
+  // offsets.
+  int32_t off32;
+  int32_t pad0;
 };
 
 // One item of a grouped launch: the contraction plus where its tiles start in

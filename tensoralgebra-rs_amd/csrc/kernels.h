@@ -26,6 +26,7 @@ enum SplitTileId : int {
   STILE_128x128 = 11,  // 4 waves of 64x64 (f32)
   STILE_64x64 = 12,    // 4 waves of 32x32
   STILE_256x128 = 13,  // 8 waves of 64x64 (f32)
+  STILE_128x64 = 14,   // 8 waves of 32x32 (c32)
 };
 
 const TileCfg* tile_cfg(int id);

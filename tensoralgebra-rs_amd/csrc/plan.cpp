@@ -113,7 +113,8 @@ Axis scaled(const Axis& a, int64_t extent, int64_t by) {
 
 bool tile_allowed(int id, DType t, int prec) {
   if (prec != PREC_STORAGE)
-    return id == STILE_64x64 || ((id == STILE_128x128 || id == STILE_256x128) && !is_complex(t));
+    return id == STILE_64x64 || (id == STILE_128x64 && is_complex(t)) ||
+           ((id == STILE_128x128 || id == STILE_256x128) && !is_complex(t));
   if (id >= 10) return false;
   if (id == TILE_128x64 && t == DType::C64) return false;  // staging would exceed 64 KiB
   return tile_cfg(id) != nullptr;
@@ -127,7 +128,7 @@ int choose_tile(const Plan& p) {
   }
   std::vector<std::pair<int, double>> cands;
   if (prec != PREC_STORAGE) {
-    cands = {{STILE_256x128, 1.3}, {STILE_128x128, 1.15}, {STILE_64x64, 1.0}};
+    cands = {{STILE_256x128, 1.3}, {STILE_128x128, 1.15}, {STILE_128x64, 1.2}, {STILE_64x64, 1.0}};
   } else {
     cands = {{TILE_128x64, 1.1}, {TILE_64x64, 1.0}, {TILE_32x32, 0.7}};
   }
@@ -231,6 +232,27 @@ void configure(Plan* plan, const PlanConfig& cfg) {
   // pointers stay valid (the kernels never index past M, N, K, H).
   for (auto* v : {&p.am, &p.cm, &p.dm, &p.bn, &p.cn, &p.dn, &p.ak, &p.bk, &p.ha, &p.hb, &p.hc, &p.hd, &p.dr})
     if (v->empty()) v->push_back(0);
+
+  auto kvec = [&](const std::vector<int64_t>& kv, const std::vector<int64_t>& rv, const std::vector<int64_t>& hv) {
+    if (p.K % 8 != 0) return false;
+    for (int64_t k = 0; k < p.K; k++)
+      if (kv[k] != k) return false;
+    const int64_t q = 16 / es;  // elements per 16 bytes
+    for (int64_t v : rv)
+      if (v % q != 0) return false;
+    for (int64_t v : hv)
+      if (v % q != 0) return false;
+    return true;
+  };
+  p.a_kvec = kvec(p.ak, p.am, p.ha);
+  p.b_kvec = kvec(p.bk, p.bn, p.hb);
+
+  auto fits32 = [&](const std::vector<int64_t>& rv, const std::vector<int64_t>& kv) {
+    const auto [rlo, rhi] = std::minmax_element(rv.begin(), rv.end());
+    const auto [klo, khi] = std::minmax_element(kv.begin(), kv.end());
+    return *rlo >= 0 && *klo >= 0 && (__int128)(*rhi + *khi + 1) * es <= (__int128)(1 << 30);
+  };
+  p.off32 = fits32(p.am, p.ak) && fits32(p.bn, p.bk);
 }
 
 Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
@@ -332,6 +354,9 @@ ContractArgs Plan::args(int device, const void* A, const void* B, const void* C,
   a.b_kmajor = b_kmajor;
   a.d_mmajor = d_mmajor;
   a.read_c = read_c;
+  a.a_kvec = a_kvec && (uintptr_t)A % 16 == 0;
+  a.b_kvec = b_kvec && (uintptr_t)B % 16 == 0;
+  a.off32 = off32;
   return a;
 }
 

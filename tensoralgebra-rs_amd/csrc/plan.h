@@ -41,6 +41,10 @@ struct Plan {
   int64_t M = 1, N = 1, K = 1, H = 1, R = 1;
   std::vector<int64_t> am, ak, bk, bn, cm, cn, dm, dn, ha, hb, hc, hd, dr;
   bool a_kmajor = true, b_kmajor = true, d_mmajor = true;
+  // Host half of ContractArgs::a_kvec / b_kvec (the pointer alignment is
+  // checked per call).
+  bool a_kvec = false, b_kvec = false;
+  bool off32 = false;  // ContractArgs::off32
   int tile = 0;
   int strategy = 0;  // 0 contraction, 1 output pass only (K empty), 2 output empty
 

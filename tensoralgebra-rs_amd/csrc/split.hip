@@ -15,9 +15,9 @@
 // Two stages, like the reference's pack-then-multiply driver
 // (crates/tprims-kernel/src/pack/pack.rs:152 packs through the scatter
 // vectors, with conjugation folded, before the microkernel runs):
-//  1. k_pack gathers each operand once through its scatter vectors (lanes
-//     along the operand's contiguous axis), splits it and writes bf16 panels
-//     in MFMA fragment order: for every (batch h, 32-wide K block, 16-row
+//  1. k_pack gathers each operand once through its scatter vectors (each
+//     lane the 8 values it later feeds the MFMA), splits it in registers and
+//     writes bf16 panels in MFMA fragment order: for every (batch h, 32-wide K block, 16-row
 //     block, component, term) one 1 KiB fragment whose 16 bytes at lane*16
 //     are exactly what lane `lane` feeds the instruction (row lane%16,
 //     k 8*(lane/16) .. +7). Splitting once per element (not once per output
@@ -59,9 +59,6 @@ __device__ inline void split(float x, __bf16* t) {
   }
 }
 
-// Position of element (row rr in 0..15, k in 0..31) inside a fragment.
-__device__ inline int frag_pos(int rr, int k) { return ((k >> 3) * 16 + rr) * 8 + (k & 7); }
-
 struct PackArgs {
   const void* X;
   const int64_t* rv;  // row scatter (length R)
@@ -70,63 +67,86 @@ struct PackArgs {
   int64_t R, K, H;
   int64_t RB;  // row blocks (of 16) per (h, K block) in the panel, padded
   int64_t KB;  // K blocks of 32
-  int32_t conj, kmajor;
+  int32_t conj;
+  int32_t vec;  // ContractArgs::a_kvec / b_kvec of this operand
   __bf16* out;
 };
 
-// One workgroup packs 64 rows x 32 k of one batch entry.
-template <bool CPLX, int TERMS>
+// Each wave packs one 16-row x 32-k block of one batch entry (4 row blocks per
+// workgroup): lane l gathers exactly the 8 values it will feed the MFMA (row
+// l % 16, k 8 (l / 16) .. + 7), splits them in registers and stores its 16
+// bytes of every component/term fragment, so each store instruction writes one
+// whole 1 KiB fragment. With `VEC` the 8 values are two (f32) or four (c32)
+// 16-byte loads per lane; otherwise one load each through the scatter vectors (rows that are
+// contiguous in memory then make each load instruction 4 runs of 16 lanes).
+template <bool CPLX, int TERMS, bool VEC>
 __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
   typedef Store<float, CPLX> St;
   typedef typename St::S S;
   constexpr int COMPS = CPLX ? 2 : 1;
   constexpr int PER = COMPS * TERMS;  // fragments per row block
-  __shared__ __attribute__((aligned(16))) __bf16 lds[4 * PER * FRAG];
-  const int tid = threadIdx.x;
-  const int64_t rgroups = p.RB / 4;  // 64-row groups (RB is a multiple of 4)
+  const int lane = threadIdx.x & 63;
+  const int64_t rgroups = p.RB / 4;  // RB is a multiple of 4
   int64_t bid = blockIdx.x;
   const int64_t rg = bid % rgroups;
   bid /= rgroups;
   const int64_t kb = bid % p.KB;
   const int64_t h = bid / p.KB;
-  const S* __restrict__ X = (const S*)p.X + p.hv[h];
-  const int64_t r0 = rg * 64, k0 = kb * KBLK;
-  const bool conj = p.conj;
+  const int64_t rb = rg * 4 + (threadIdx.x >> 6);
+  const int64_t r = rb * 16 + (lane & 15);
+  const int64_t k0 = kb * KBLK + 8 * (lane >> 4);
+  const bool rok = r < p.R;
+  // Row 0 and k 0 have offset 0, so X (and X + 0..7 under VEC) is always a
+  // valid address of this batch entry; out-of-range values are read from
+  // there and replaced by zero.
+  const S* __restrict__ X = (const S*)p.X + p.hv[h] + (rok ? p.rv[r] : 0);
+  float re[8], im[8];
+  if constexpr (VEC) {
+    const bool ok = rok && k0 < p.K;  // K % 8 == 0: all 8 k in range or none
+    const v4f* src = (const v4f*)(X + (ok ? k0 : 0));
+    if constexpr (!CPLX) {
+      const v4f x0 = src[0], x1 = src[1];
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    int rl, kl;
-    if (p.kmajor) {
-      kl = tid & 31;
-      rl = (tid >> 5) + 8 * i;
+      for (int j = 0; j < 4; j++) {
+        re[j] = ok ? x0[j] : 0.0f;
+        re[4 + j] = ok ? x1[j] : 0.0f;
+        im[j] = im[4 + j] = 0.0f;
+      }
     } else {
-      rl = tid & 63;
-      kl = (tid >> 6) + 4 * i;
+      v4f x[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) x[c] = src[c];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        re[j] = ok ? x[j >> 1][2 * (j & 1)] : 0.0f;
+        im[j] = ok ? x[j >> 1][2 * (j & 1) + 1] : 0.0f;
+      }
     }
-    const int64_t r = r0 + rl, k = k0 + kl;
-    const bool ok = r < p.R && k < p.K;
-    const S v = X[ok ? p.rv[r] + p.kv[k] : 0];
-    const float re = ok ? St::re(v) : 0.0f;
-    const float im0 = ok ? St::im(v) : 0.0f;
-    const float im = conj ? -im0 : im0;
-    __bf16 t[TERMS];
-    const int pos = frag_pos(rl & 15, kl);
-    const int base = (rl >> 4) * PER * FRAG;
-    split<TERMS>(re, t);
+  } else {
 #pragma unroll
-    for (int x = 0; x < TERMS; x++) lds[base + x * FRAG + pos] = t[x];
-    if constexpr (CPLX) {
-      split<TERMS>(im, t);
-#pragma unroll
-      for (int x = 0; x < TERMS; x++) lds[base + (TERMS + x) * FRAG + pos] = t[x];
+    for (int j = 0; j < 8; j++) {
+      const int64_t k = k0 + j;
+      const bool ok = rok && k < p.K;
+      const S v = X[ok ? p.kv[k] : 0];
+      re[j] = ok ? St::re(v) : 0.0f;
+      im[j] = ok ? St::im(v) : 0.0f;
     }
   }
-  __syncthreads();
-  // The 4 row blocks are contiguous in the panel: copy 16 bytes per thread.
-  v4u* dst = (v4u*)(p.out + ((h * p.KB + kb) * p.RB + rg * 4) * PER * FRAG);
-  const v4u* src = (const v4u*)lds;
-  constexpr int CHUNKS = 4 * PER * FRAG * 2 / 16;
+  __bf16* out = p.out + ((h * p.KB + kb) * p.RB + rb) * PER * FRAG + lane * 8;
 #pragma unroll
-  for (int c = tid; c < CHUNKS; c += 256) dst[c] = src[c];
+  for (int c = 0; c < COMPS; c++) {
+    v8bf t[TERMS];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const float v = c == 0 ? re[j] : (p.conj ? -im[j] : im[j]);
+      __bf16 s[TERMS];
+      split<TERMS>(v, s);
+#pragma unroll
+      for (int x = 0; x < TERMS; x++) t[x][j] = s[x];
+    }
+#pragma unroll
+    for (int x = 0; x < TERMS; x++) *(v8bf*)(out + (c * TERMS + x) * FRAG) = t[x];
+  }
 }
 
 template <bool CPLX_, int TERMS_, int BM_, int BN_, int WM_, int WN_, int STAGES_>
@@ -215,50 +235,82 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
 #pragma unroll
       for (int j = 0; j < G::NI; j++) acc[s][i][j] = v4f{0, 0, 0, 0};
 
-  // Pipeline: K block kb lives in stage kb % STAGES. Before computing block kb
-  // a wave waits for its own copies of kb (vmcnt), the barrier publishes
-  // everyone's and guarantees every wave has finished block kb - 1, whose
-  // stage then receives block kb + STAGES - 1.
+  // Fragment (row block blk, component c, term x) of a stage; B's follow A's.
+  auto frag = [&](const char* stage, int blk, int c, int x) -> v8bf {
+    return *(const v8bf*)((const __bf16*)stage + ((size_t)(blk * PER + c * G::TERMS + x) * FRAG + lane * 8));
+  };
+  v8bf a[G::COMPS][G::TERMS][G::MI], b[G::COMPS][G::TERMS][G::NI], nb[G::COMPS][G::TERMS][G::NI];
+
+  // Two stages; K block kb lives in stage kb & 1. Registers are one block
+  // ahead of the MFMAs: while block kb is multiplied, block kb + 1 is read
+  // from LDS (B into nb up front, A's row block i as soon as the MFMAs of
+  // block kb no longer need it), so LDS reads overlap the matrix cores.
+  // At the top of iteration kb every wave has its own copies of block kb + 1
+  // landed (vmcnt) and its reads of block kb done (lgkmcnt); after the
+  // barrier block kb + 1 is visible to all and nobody reads stage kb & 1
+  // again, which then receives block kb + 2.
   const int64_t nk = g.KB;
+  issue(0, 0);
+  if (nk > 1) {
+    issue(1, 1);
+    wait_vmcnt<G::PW>();
+  } else {
+    wait_vmcnt<0>();
+  }
+  __syncthreads();
 #pragma unroll
-  for (int s = 0; s < G::STAGES - 1; s++)
-    if (s < nk) issue(s, s);
-  for (int64_t kb = 0; kb < nk; kb++) {
-    if constexpr (G::STAGES == 3) {
-      if (kb + 1 < nk) wait_vmcnt<G::PW>();
-      else wait_vmcnt<0>();
-    } else {
-      wait_vmcnt<0>();
+  for (int c = 0; c < G::COMPS; c++)
+#pragma unroll
+    for (int x = 0; x < G::TERMS; x++) {
+#pragma unroll
+      for (int i = 0; i < G::MI; i++) a[c][x][i] = frag(smem, (wm >> 4) + i, c, x);
+#pragma unroll
+      for (int j = 0; j < G::NI; j++) b[c][x][j] = frag(smem + G::A_FRAGS * FRAG * 2, (wn >> 4) + j, c, x);
     }
+  for (int64_t kb = 0; kb < nk; kb++) {
+    // vmcnt(0) and lgkmcnt(0): own copies landed, own LDS reads complete.
+    __builtin_amdgcn_s_waitcnt(7 << 4);
     __syncthreads();
-    if (kb + G::STAGES - 1 < nk) issue(kb + G::STAGES - 1, (int)((kb + G::STAGES - 1) % G::STAGES));
-    const __bf16* As = (const __bf16*)(smem + (size_t)(kb % G::STAGES) * G::STAGE_BYTES);
-    const __bf16* Bs = As + G::A_FRAGS * FRAG;
+    if (kb + 2 < nk) issue(kb + 2, (int)(kb & 1));
+    // Block kb + 1 (stale data in the last iteration, never used).
+    const char* An = smem + (size_t)((kb + 1) & 1) * G::STAGE_BYTES;
+    const char* Bn = An + G::A_FRAGS * FRAG * 2;
 #pragma unroll
-    for (int c = 0; c < G::COMPS; c++) {
+    for (int c = 0; c < G::COMPS; c++)
 #pragma unroll
-      for (int d = 0; d < G::COMPS; d++) {
-        const int slot = !G::CPLX ? 0 : (c == 0 && d == 0 ? 0 : (c == 1 && d == 1 ? 1 : 2));
-        v8bf a[G::TERMS][G::MI], b[G::TERMS][G::NI];
+      for (int x = 0; x < G::TERMS; x++)
 #pragma unroll
-        for (int x = 0; x < G::TERMS; x++) {
+        for (int j = 0; j < G::NI; j++) nb[c][x][j] = frag(Bn, (wn >> 4) + j, c, x);
+    // Scheduling fences keep the reads where they are written (the compiler
+    // would otherwise sink them below the MFMAs, serialising LDS and MFMA).
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int i = 0; i < G::MI; i++)
-            a[x][i] = *(const v8bf*)&As[(((wm >> 4) + i) * PER + c * G::TERMS + x) * FRAG + lane * 8];
+    for (int i = 0; i < G::MI; i++) {
 #pragma unroll
-          for (int j = 0; j < G::NI; j++)
-            b[x][j] = *(const v8bf*)&Bs[(((wn >> 4) + j) * PER + d * G::TERMS + x) * FRAG + lane * 8];
-        }
+      for (int c = 0; c < G::COMPS; c++)
 #pragma unroll
-        for (int pp = 0; pp < NP; pp++)
+        for (int d = 0; d < G::COMPS; d++) {
+          const int slot = !G::CPLX ? 0 : (c == 0 && d == 0 ? 0 : (c == 1 && d == 1 ? 1 : 2));
 #pragma unroll
-          for (int i = 0; i < G::MI; i++)
+          for (int pp = 0; pp < NP; pp++)
 #pragma unroll
             for (int j = 0; j < G::NI; j++)
-              acc[slot][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[PA[pp]][i], b[PB[pp]][j], acc[slot][i][j],
-                                                                        0, 0, 0);
-      }
+              acc[slot][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[c][PA[pp]][i], b[d][PB[pp]][j],
+                                                                        acc[slot][i][j], 0, 0, 0);
+        }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int c = 0; c < G::COMPS; c++)
+#pragma unroll
+        for (int x = 0; x < G::TERMS; x++) a[c][x][i] = frag(An, (wm >> 4) + i, c, x);
+      __builtin_amdgcn_sched_barrier(0);
     }
+#pragma unroll
+    for (int c = 0; c < G::COMPS; c++)
+#pragma unroll
+      for (int x = 0; x < G::TERMS; x++)
+#pragma unroll
+        for (int j = 0; j < G::NI; j++) b[c][x][j] = nb[c][x][j];
   }
   __syncthreads();  // every wave is done with the stages before they become the staging tile
 
@@ -290,6 +342,7 @@ struct Shape {
 Shape shape_of(int tile_id, bool cplx) {
   if (tile_id == STILE_256x128 && !cplx) return {256, 128};
   if (tile_id == STILE_128x128 && !cplx) return {128, 128};
+  if (tile_id == STILE_128x64 && cplx) return {128, 64};
   return {64, 64};
 }
 
@@ -312,12 +365,13 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
   const int64_t RBa = pad(a.M, sh.bm), RBb = pad(a.N, sh.bn);
   __bf16* Ap = (__bf16*)panels;
   __bf16* Bp = Ap + a.H * KB * RBa * PER * FRAG;
-  PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kmajor, Ap};
-  PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kmajor, Bp};
+  PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kvec, Ap};
+  PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, Bp};
   for (const PackArgs* q : {&pa, &pb}) {
     const int64_t grid = q->H * q->KB * (q->RB / 4);
     if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_pack<CPLX, TERMS>), dim3((unsigned)grid), dim3(256), 0, s, *q);
+    if (q->vec) hipLaunchKernelGGL((k_pack<CPLX, TERMS, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
+    else hipLaunchKernelGGL((k_pack<CPLX, TERMS, false>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -331,7 +385,8 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
     else if (sh.bm == 128) launch_gemm<GCfg<CPLX, TERMS, 128, 128, 64, 64, 2>>(g, nwg, s);
     else launch_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32, 2>>(g, nwg, s);
   } else {
-    launch_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32, 2>>(g, nwg, s);
+    if (sh.bm == 128) launch_gemm<GCfg<CPLX, TERMS, 128, 64, 32, 32, 2>>(g, nwg, s);
+    else launch_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32, 2>>(g, nwg, s);
   }
   return hipGetLastError();
 }
