@@ -54,6 +54,11 @@ struct Mma<double> {
   __device__ static inline acc run(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // c - a b: the f64 instructions read their last field as negate flags
+  // (bit 0 negates A), so the negation costs nothing.
+  __device__ static inline acc run_neg_a(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
+  }
   __device__ static inline int row(int lk, int q) { return lk + 4 * q; }
 };
 template <>
@@ -77,18 +82,23 @@ struct Cfg {
   static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
   static constexpr int WAVES_M = BM / WM;
   static constexpr int NT = (BM / WM) * (BN / WN) * 64;
-  static constexpr int PLANES = METHOD == M_REAL ? 1 : (METHOD == M_3M ? 3 : 2);
-  // Plane layouts: K-major gathers store [m][k], M-major gathers [k][m] with
-  // pitch BM+16. The K-major pitch puts the 16 rows of a fragment read on
-  // distinct banks of the 64 x 4-byte LDS banks: an odd pitch (BK+1 words)
-  // for f32; for f64 BK+2 doubles (a multiple of 4 words that is an odd
-  // multiple of 4 modulo 64), so that a half-wave's 32 eight-byte reads (16
-  // rows x 2 k) cover all 64 banks exactly once.
-  static constexpr int KPAD = sizeof(T) == 8 ? 2 : 1;
+  // The native complex schedule keeps re and im of an element together (one
+  // 8- or 16-byte LDS element, one store and one read each); 1m and 3m use
+  // separate planes of T.
+  static constexpr bool ILV = METHOD == M_NATIVE;
+  static constexpr int EW = ILV ? 2 : 1;  // T values per LDS element
+  static constexpr int PLANES = METHOD == M_REAL || ILV ? 1 : (METHOD == M_3M ? 3 : 2);
+  // Plane layouts (in elements): K-major gathers store [m][k], M-major
+  // gathers [k][m] with pitch BM+16. The K-major pitch puts the rows of a
+  // fragment read on distinct banks of the 64 x 4-byte LDS banks: odd (BK+1)
+  // for 4- and 16-byte elements (16 rows of one k in one pass); BK+2 for
+  // 8-byte elements (twice an odd number, so a pass's 32 reads, 16 rows x 2 k,
+  // cover all 64 banks exactly once).
+  static constexpr int KPAD = EW * sizeof(T) == 8 ? 2 : 1;
   static constexpr int LDAK = BK + KPAD, LDAM = BM + 16;
   static constexpr int LDBK = BK + KPAD, LDBN = BN + 16;
-  static constexpr int A_PLANE = AKM ? BM * LDAK : BK * LDAM;
-  static constexpr int B_PLANE = BKM ? BN * LDBK : BK * LDBN;
+  static constexpr int A_PLANE = EW * (AKM ? BM * LDAK : BK * LDAM);  // in T
+  static constexpr int B_PLANE = EW * (BKM ? BN * LDBK : BK * LDBN);
   static constexpr int LDD = BN + 1;  // row pitch of the staging tile
   static constexpr int EA = BM * BK / NT;
   static constexpr int EB = BN * BK / NT;
@@ -97,8 +107,12 @@ struct Cfg {
   static constexpr int RA = AKM ? EA : 1;  // row offsets a thread tracks for A
   static constexpr int RB = BKM ? EB : 1;
   static constexpr int MI = WM / 16, NI = WN / 16;
-  static constexpr int ACCS = METHOD == M_REAL ? 1 : (METHOD == M_3M ? 3 : 2);
-  static constexpr int STAGE = PLANES * (A_PLANE + B_PLANE);  // elements of one LDS stage
+  // Accumulators: real 1; native 2 (re, im) for f64, whose MFMA negates an
+  // operand for free, 3 (re*re, im*im, cross) for f32; 1m 1; 3m 3.
+  static constexpr bool NEG_MFMA = sizeof(T) == 8;
+  static constexpr int ACCS =
+      METHOD == M_REAL || METHOD == M_1M ? 1 : (METHOD == M_NATIVE && NEG_MFMA ? 2 : 3);
+  static constexpr int STAGE = PLANES * (A_PLANE + B_PLANE);  // T values of one LDS stage
   static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * sizeof(T);
   static constexpr size_t LDS_EPI = (size_t)(CPLX ? 2 : 1) * BM * LDD * sizeof(T);
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
@@ -229,8 +243,15 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
 #pragma unroll
     for (int i = 0; i < EA; i++) {
       const bool ok = C::BUF || ((a_live >> i) & 1);  // BUF: out-of-range loads read zero
-      T* dst = As + C::aidx(a_m0 + i * a_mstep, a_k0 + i * a_kstep);
+      const int e = C::aidx(a_m0 + i * a_mstep, a_k0 + i * a_kstep);
       const T re = ok ? St::re(ra[i]) : T(0);
+      if constexpr (C::ILV) {
+        T im = ok ? St::im(ra[i]) : T(0);
+        if (ca) im = -im;
+        ((dev::Cx<T>*)As)[e] = dev::Cx<T>{re, im};
+        continue;
+      }
+      T* dst = As + e;
       dst[0] = re;
       if constexpr (C::CPLX) {
         T im = ok ? St::im(ra[i]) : T(0);
@@ -242,8 +263,15 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
 #pragma unroll
     for (int i = 0; i < EB; i++) {
       const bool ok = C::BUF || ((b_live >> i) & 1);
-      T* dst = Bs + C::bidx(b_n0 + i * b_nstep, b_k0 + i * b_kstep);
+      const int e = C::bidx(b_n0 + i * b_nstep, b_k0 + i * b_kstep);
       const T re = ok ? St::re(rb[i]) : T(0);
+      if constexpr (C::ILV) {
+        T im = ok ? St::im(rb[i]) : T(0);
+        if (cb) im = -im;
+        ((dev::Cx<T>*)Bs)[e] = dev::Cx<T>{re, im};
+        continue;
+      }
+      T* dst = Bs + e;
       dst[0] = re;
       if constexpr (C::CPLX) {
         T im = ok ? St::im(rb[i]) : T(0);
@@ -296,31 +324,37 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
           for (int j = 0; j < C::NI; j++) acc[0][i][j] = MM::run(a[i], b[j], acc[0][i][j]);
       } else if constexpr (C::METH == M_NATIVE) {
         T ar[C::MI], ai[C::MI], br[C::NI], bi[C::NI];
+        const dev::Cx<T>* Ac = (const dev::Cx<T>*)As;
+        const dev::Cx<T>* Bc = (const dev::Cx<T>*)Bs;
 #pragma unroll
         for (int i = 0; i < C::MI; i++) {
-          const int x = C::aidx(wm + 16 * i + lr, kk + lk);
-          ar[i] = As[x];
-          ai[i] = As[C::A_PLANE + x];
+          const dev::Cx<T> v = Ac[C::aidx(wm + 16 * i + lr, kk + lk)];
+          ar[i] = v.re;
+          ai[i] = v.im;
         }
 #pragma unroll
         for (int j = 0; j < C::NI; j++) {
-          const int x = C::bidx(wn + 16 * j + lr, kk + lk);
-          br[j] = Bs[x];
-          bi[j] = Bs[C::B_PLANE + x];
+          const dev::Cx<T> v = Bc[C::bidx(wn + 16 * j + lr, kk + lk)];
+          br[j] = v.re;
+          bi[j] = v.im;
         }
+        // f64: acc0 += ar br - ai bi, acc1 += ar bi + ai br. f32: acc0 += ar br,
+        // acc1 += ai bi (subtracted at the end), acc2 += ar bi + ai br.
+        constexpr int XI = C::NEG_MFMA ? 1 : 2;
 #pragma unroll
         for (int i = 0; i < C::MI; i++)
 #pragma unroll
           for (int j = 0; j < C::NI; j++) {
             acc[0][i][j] = MM::run(ar[i], br[j], acc[0][i][j]);
-            acc[1][i][j] = MM::run(ar[i], bi[j], acc[1][i][j]);
+            acc[XI][i][j] = MM::run(ar[i], bi[j], acc[XI][i][j]);
           }
 #pragma unroll
         for (int i = 0; i < C::MI; i++)
 #pragma unroll
           for (int j = 0; j < C::NI; j++) {
-            acc[0][i][j] = MM::run(-ai[i], bi[j], acc[0][i][j]);
-            acc[1][i][j] = MM::run(ai[i], br[j], acc[1][i][j]);
+            if constexpr (C::NEG_MFMA) acc[0][i][j] = MM::run_neg_a(ai[i], bi[j], acc[0][i][j]);
+            else acc[1][i][j] = MM::run(ai[i], bi[j], acc[1][i][j]);
+            acc[XI][i][j] = MM::run(ai[i], br[j], acc[XI][i][j]);
           }
       } else if constexpr (C::METH == M_3M) {
         T a[3][C::MI], b[3][C::NI];
@@ -394,9 +428,12 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
           const int row = wm + 16 * i + rr;
           if constexpr (C::METH == M_REAL) {
             Dre[row * C::LDD + col] = acc[0][i][j][q];
-          } else if constexpr (C::METH == M_NATIVE) {
+          } else if constexpr (C::METH == M_NATIVE && C::NEG_MFMA) {
             Dre[row * C::LDD + col] = acc[0][i][j][q];
             Dim[row * C::LDD + col] = acc[1][i][j][q];
+          } else if constexpr (C::METH == M_NATIVE) {
+            Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
+            Dim[row * C::LDD + col] = acc[2][i][j][q];
           } else {
             const T m1 = acc[0][i][j][q], m2 = acc[1][i][j][q], m3 = acc[2][i][j][q];
             Dre[row * C::LDD + col] = m1 - m2;
