@@ -39,36 +39,7 @@ namespace {
 
 using dev::Store;
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-// Output placement of one 16x16 accumulator block: lane (lk = lane / 16,
-// column lane % 16) holds four rows. The f64 instruction interleaves them
-// (row lk + 4q); the f32 one keeps them together (row 4 lk + q). Both were
-// measured on gfx950 with known inputs.
-template <typename T>
-struct Mma;
-template <>
-struct Mma<double> {
-  typedef v4d acc;
-  __device__ static inline acc run(double a, double b, acc c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  // c - a b: the f64 instructions read their last field as negate flags
-  // (bit 0 negates A), so the negation costs nothing.
-  __device__ static inline acc run_neg_a(double a, double b, acc c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
-  }
-  __device__ static inline int row(int lk, int q) { return lk + 4 * q; }
-};
-template <>
-struct Mma<float> {
-  typedef v4f acc;
-  __device__ static inline acc run(float a, float b, acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  __device__ static inline int row(int lk, int q) { return 4 * lk + q; }
-};
+using dev::Mma;
 
 enum { M_REAL = -1, M_NATIVE = 0, M_1M = 1, M_3M = 2 };
 
@@ -602,6 +573,7 @@ const TileCfg* tile_cfg(int id) {
 }
 
 hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s) {
+  if (dma_supported(a.off32, t, method, tile_id)) return launch_contract_dma(a, t, tile_id, s);
   switch (t) {
     case DType::F64: return dispatch_tile<double, M_REAL>(a, tile_id, s);
     case DType::F32: return dispatch_tile<float, M_REAL>(a, tile_id, s);

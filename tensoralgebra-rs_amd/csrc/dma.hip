@@ -1,0 +1,318 @@
+// Storage-precision contraction kernels whose gathers are LDS DMA: every
+// lane's buffer load writes its element straight into LDS (buffer_load ... lds),
+// so the slabs never pass through registers and no VALU or LDS store
+// instruction is spent on them. Selected for f32 when every gather offset
+// fits the 32-bit buffer range (ContractArgs::off32); the template also takes
+// c64 with the native schedule, which is not selected (dma_supported). Other
+// cases run the register-staged kernels of contract.hip, with the same
+// semantics (crates/tprims-kernel/src/pack/pack.rs:57-265 packing,
+// kernels/reference/scalar.rs:31,64 microkernels, pack/writeback.rs:114-495
+// write-back, driver/tile.rs:142 K accumulation).
+//
+// A DMA instruction writes lane i's element at LDS slot i of a 64-element
+// block, so the LDS layout is fixed by which element each lane fetches:
+//  - K-major operands (contiguous along K): a block holds RPI = 64 / BK rows
+//    x BK k; lane i fetches row i / BK, k (i % BK) ^ (SW * row). Blocks sit
+//    PADB elements apart. PADB and SW make the MFMA fragment reads (16 rows x
+//    4 k for 4-byte elements, 16 rows x 1 k per pass for 16-byte ones) hit 64
+//    distinct banks: f32 PADB 4, SW 0; c64 PADB 1, SW 4.
+//  - M-major operands: one row of BM elements per k, lanes along M, pitch
+//    BM + 16 for f32 (lanes of 4 k land on distinct bank quarters).
+// Out-of-range rows and k use the buffer sentinels and arrive as zeros.
+// Conjugation is applied to the imaginary parts as they are read for the MFMA
+// (a sign flip of the high word), the f64 MFMA negate flag forms -im*im.
+//
+// Pipeline over two LDS stages, one barrier per K slab: MFMAs on slab kt,
+// wait for this wave's DMA of slab kt + 1, barrier (slab kt + 1 visible,
+// stage kt & 1 retired), then the DMA of slab kt + 2 into stage kt & 1.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace tp {
+namespace {
+
+using dev::Cx;
+using dev::Mma;
+using dev::Store;
+
+// One LDS-DMA instruction: every lane's ES bytes at byte offset `off` of the
+// resource go to LDS at dst + 16 lane (ES 16) or dst + 4 lane (ES 4). The
+// builtin is only visible to the device pass: the host pass rejects the
+// 16-byte size for its own target and would then drop the kernel stubs.
+template <int ES>
+__device__ inline void dma_load(__amdgpu_buffer_rsrc_t r, void* dst, uint32_t off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (ES == 16) __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 16, off, 0, 0, 0);
+  else __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 4, off, 0, 0, 0);
+#endif
+}
+
+template <typename T, bool CPLX_, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_>
+struct DCfg {
+  typedef T Tv;
+  static constexpr bool CPLX = CPLX_, AKM = AKM_, BKM = BKM_;
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+  static constexpr int WAVES_M = BM / WM;
+  static constexpr int NW = (BM / WM) * (BN / WN);
+  static constexpr int NT = NW * 64;
+  typedef typename Store<T, CPLX>::S S;
+  static constexpr int ES = sizeof(S);  // 4 (f32) or 16 (c64): the DMA sizes used
+  static constexpr int RPI = 64 / BK;   // rows per K-major block
+  static constexpr int PADB = ES == 16 ? 1 : 4;
+  static constexpr int SW = ES == 16 ? 4 : 0;
+  static constexpr int PADM = ES == 16 ? 0 : 16;
+  static constexpr int A_ELEMS = AKM ? (BM / RPI) * (64 + PADB) : BK * (BM + PADM);
+  static constexpr int B_ELEMS = BKM ? (BN / RPI) * (64 + PADB) : BK * (BN + PADM);
+  static constexpr int STAGE = A_ELEMS + B_ELEMS;  // elements of one stage
+  static constexpr int A_INSTR = BM * BK / 64, B_INSTR = BN * BK / 64;
+  static constexpr int JA = A_INSTR / NW, JB = B_INSTR / NW;  // DMA instructions per wave per slab
+  static constexpr int MI = WM / 16, NI = WN / 16;
+  static constexpr int ACCS = CPLX ? 2 : 1;
+  static constexpr int LDD = BN + 1;
+  static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * ES;
+  static constexpr size_t LDS_EPI = (size_t)(CPLX ? 2 : 1) * BM * LDD * sizeof(T);
+  static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
+  static_assert(ES == 4 || ES == 16, "LDS DMA moves 4 or 16 bytes per lane");
+  static_assert(64 % BK == 0 && BM % RPI == 0 && BN % RPI == 0, "K-major blocks");
+  static_assert(BM % 64 == 0 || AKM, "M-major rows are whole DMA instructions");
+  static_assert(BN % 64 == 0 || BKM, "N-major rows are whole DMA instructions");
+  static_assert(A_INSTR % NW == 0 && B_INSTR % NW == 0, "whole DMA instructions per wave");
+  static_assert(BK % 4 == 0 && WM % 16 == 0 && WN % 16 == 0, "MFMA shape");
+  static_assert(!CPLX || sizeof(T) == 8, "complex DMA path is c64 (MFMA negate flag)");
+  static_assert(LDS <= 160 * 1024, "LDS budget");
+
+  // LDS slot of element (row r, k) of a stage's A (rows BM) or B (rows BN) part.
+  template <bool KM, int ROWS>
+  __device__ static inline int slot(int r, int k) {
+    if constexpr (KM) return (r / RPI) * (64 + PADB) + (r % RPI) * BK + (k ^ (SW * (r % RPI)));
+    else return k * (ROWS + PADM) + r;
+  }
+};
+
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
+  typedef typename C::Tv T;
+  typedef typename C::S S;
+  typedef Mma<T> MM;
+  typedef typename MM::acc Acc;
+  constexpr int BM = C::BM, BN = C::BN, BK = C::BK, NW = C::NW;
+  __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+  S* const lds = (S*)smem;
+
+  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
+  int64_t h;
+  int tm, tn;
+  dev::decode_tile(blockIdx.x, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = (wave % C::WAVES_M) * C::WM;
+  const int wn = (wave / C::WAVES_M) * C::WN;
+  const int64_t M = p.M, N = p.N, K = p.K;
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const __amdgpu_buffer_rsrc_t rsA = dev::make_rsrc((const S*)p.A + p.ha[h]);
+  const __amdgpu_buffer_rsrc_t rsB = dev::make_rsrc((const S*)p.B + p.hb[h]);
+
+  // What each lane fetches: its row (fixed per instruction) and its k within
+  // the slab (K-major: fixed per lane; M-major: fixed per instruction).
+  uint32_t a_row[C::JA], b_row[C::JB];
+  int a_kl[C::JA], b_kl[C::JB];
+#pragma unroll
+  for (int t = 0; t < C::JA; t++) {
+    const int j = wave + NW * t;
+    int r, kl;
+    if constexpr (C::AKM) {
+      r = j * C::RPI + lane / BK;
+      kl = (lane % BK) ^ (C::SW * (lane / BK));
+    } else {
+      r = (j % (BM / 64)) * 64 + lane;
+      kl = j / (BM / 64);
+    }
+    const int64_t m = m0 + r;
+    a_row[t] = m < M ? (uint32_t)(p.am[m] * C::ES) : dev::ROW_OOB;
+    a_kl[t] = kl;
+  }
+#pragma unroll
+  for (int t = 0; t < C::JB; t++) {
+    const int j = wave + NW * t;
+    int r, kl;
+    if constexpr (C::BKM) {
+      r = j * C::RPI + lane / BK;
+      kl = (lane % BK) ^ (C::SW * (lane / BK));
+    } else {
+      r = (j % (BN / 64)) * 64 + lane;
+      kl = j / (BN / 64);
+    }
+    const int64_t n = n0 + r;
+    b_row[t] = n < N ? (uint32_t)(p.bn[n] * C::ES) : dev::ROW_OOB;
+    b_kl[t] = kl;
+  }
+  // LDS destination of instruction t of this wave within a stage.
+  auto a_dst = [&](int t) {
+    const int j = wave + NW * t;
+    return C::AKM ? j * (64 + C::PADB) : (j / (BM / 64)) * (BM + C::PADM) + (j % (BM / 64)) * 64;
+  };
+  auto b_dst = [&](int t) {
+    const int j = wave + NW * t;
+    return C::A_ELEMS + (C::BKM ? j * (64 + C::PADB) : (j / (BN / 64)) * (BN + C::PADM) + (j % (BN / 64)) * 64);
+  };
+
+  // K byte offsets of one slab from the padded 32-bit K scatters (the
+  // sentinel past K), plain loads whose first use is the DMA issue.
+  auto load_koff = [&](int64_t k0, uint32_t* ka, uint32_t* kb) {
+#pragma unroll
+    for (int t = 0; t < C::JA; t++) ka[t] = p.ak32[k0 + a_kl[t]];
+#pragma unroll
+    for (int t = 0; t < C::JB; t++) kb[t] = p.bk32[k0 + b_kl[t]];
+  };
+  auto issue = [&](int st, const uint32_t* ka, const uint32_t* kb) {
+    S* stage = lds + st * C::STAGE;
+#pragma unroll
+    for (int t = 0; t < C::JA; t++) dma_load<C::ES>(rsA, stage + a_dst(t), a_row[t] + ka[t]);
+#pragma unroll
+    for (int t = 0; t < C::JB; t++) dma_load<C::ES>(rsB, stage + b_dst(t), b_row[t] + kb[t]);
+  };
+
+  Acc acc[C::ACCS][C::MI][C::NI];
+#pragma unroll
+  for (int s = 0; s < C::ACCS; s++)
+#pragma unroll
+    for (int i = 0; i < C::MI; i++)
+#pragma unroll
+      for (int j = 0; j < C::NI; j++) acc[s][i][j] = Acc{};
+
+  const int lr = lane & 15, lk = lane >> 4;
+  const uint32_t ca = p.conj_a ? 0x80000000u : 0u, cb = p.conj_b ? 0x80000000u : 0u;
+  auto flip = [](T x, uint32_t sign) {
+    if constexpr (sizeof(T) == 8) {
+      return __builtin_bit_cast(T, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)sign << 32));
+    } else {
+      return x;
+    }
+  };
+  auto compute = [&](int st) {
+    const S* As = lds + st * C::STAGE;
+    const S* Bs = As + C::A_ELEMS;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      if constexpr (!C::CPLX) {
+        T a[C::MI], b[C::NI];
+#pragma unroll
+        for (int i = 0; i < C::MI; i++) a[i] = As[C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk)];
+#pragma unroll
+        for (int j = 0; j < C::NI; j++) b[j] = Bs[C::template slot<C::BKM, BN>(wn + 16 * j + lr, kk + lk)];
+#pragma unroll
+        for (int i = 0; i < C::MI; i++)
+#pragma unroll
+          for (int j = 0; j < C::NI; j++) acc[0][i][j] = MM::run(a[i], b[j], acc[0][i][j]);
+      } else {
+        T ar[C::MI], ai[C::MI], br[C::NI], bi[C::NI];
+#pragma unroll
+        for (int i = 0; i < C::MI; i++) {
+          const S v = As[C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk)];
+          ar[i] = v.re;
+          ai[i] = flip(v.im, ca);
+        }
+#pragma unroll
+        for (int j = 0; j < C::NI; j++) {
+          const S v = Bs[C::template slot<C::BKM, BN>(wn + 16 * j + lr, kk + lk)];
+          br[j] = v.re;
+          bi[j] = flip(v.im, cb);
+        }
+        // acc0 += ar br - ai bi, acc1 += ar bi + ai br
+#pragma unroll
+        for (int i = 0; i < C::MI; i++)
+#pragma unroll
+          for (int j = 0; j < C::NI; j++) {
+            acc[0][i][j] = MM::run(ar[i], br[j], acc[0][i][j]);
+            acc[1][i][j] = MM::run(ar[i], bi[j], acc[1][i][j]);
+          }
+#pragma unroll
+        for (int i = 0; i < C::MI; i++)
+#pragma unroll
+          for (int j = 0; j < C::NI; j++) {
+            acc[0][i][j] = MM::run_neg_a(ai[i], bi[j], acc[0][i][j]);
+            acc[1][i][j] = MM::run(ai[i], br[j], acc[1][i][j]);
+          }
+      }
+    }
+  };
+
+  const int64_t nk = (K + BK - 1) / BK;
+  uint32_t a_ko[C::JA], b_ko[C::JB];
+  {
+    uint32_t a1[C::JA], b1[C::JB];
+    load_koff(0, a_ko, b_ko);
+    if (nk > 1) load_koff(BK, a1, b1);
+    issue(0, a_ko, b_ko);
+    if (nk > 1) issue(1, a1, b1);
+  }
+  __builtin_amdgcn_s_waitcnt(7 << 4);  // vmcnt(0) lgkmcnt(0)
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; kt++) {
+    const bool more = kt + 2 < nk;
+    if (more) load_koff((kt + 2) * BK, a_ko, b_ko);  // lands during the MFMAs
+    compute((int)(kt & 1));
+    __builtin_amdgcn_s_waitcnt(7 << 4);  // this wave's DMA of slab kt + 1 (and the offsets)
+    __syncthreads();
+    if (more) issue((int)(kt & 1), a_ko, b_ko);
+  }
+
+  // Stage the tile (Dre/Dim [BM][LDD]) and write it back.
+  T* Dre = (T*)smem;
+  T* Dim = Dre + BM * C::LDD;
+#pragma unroll
+  for (int i = 0; i < C::MI; i++)
+#pragma unroll
+    for (int j = 0; j < C::NI; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = wm + 16 * i + MM::row(lk, q), col = wn + 16 * j + lr;
+        Dre[row * C::LDD + col] = acc[0][i][j][q];
+        if constexpr (C::CPLX) Dim[row * C::LDD + col] = acc[1][i][j][q];
+      }
+  __syncthreads();
+  dev::write_back<T, T, C::CPLX, BM, BN, C::NT, C::LDD>(p, h, m0, n0, Dre, Dim);
+}
+
+template <class C>
+hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
+  ContractArgs p = a;
+  p.tiles_m = (int)((p.M + C::BM - 1) / C::BM);
+  p.tiles_n = (int)((p.N + C::BN - 1) / C::BN);
+  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
+  if (nwg == 0) return hipSuccess;
+  if (nwg > 0x7fffffff) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(k_contract_dma<C>, dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN>
+hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
+  if (a.a_kmajor) {
+    if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true>>(a, s);
+    return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false>>(a, s);
+  }
+  if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true>>(a, s);
+  return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false>>(a, s);
+}
+
+}  // namespace
+
+// f32 only: for c64 (native, 64x64) the DMA variant measured slower than the
+// register-staged kernel on the complex benchmark (2.44-2.50 ms against 2.35
+// ms), so the planner keeps c64 on the latter.
+bool dma_supported(bool off32, DType t, int method, int tile_id) {
+  (void)method;
+  if (!off32) return false;
+  return t == DType::F32 && (tile_id == TILE_128x64 || tile_id == TILE_64x64);
+}
+
+hipError_t launch_contract_dma(const ContractArgs& a, DType t, int tile_id, hipStream_t s) {
+  if (t != DType::F32) return hipErrorInvalidValue;
+  if (tile_id == TILE_128x64) return launch_majors<float, false, 128, 64, 16, 64, 32>(a, s);
+  return launch_majors<float, false, 64, 64, 16, 32, 32>(a, s);
+}
+
+}  // namespace tp

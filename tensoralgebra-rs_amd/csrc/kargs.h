@@ -47,6 +47,10 @@ struct ContractArgs {
   // offsets.
   int32_t off32;
   int32_t pad0;
+  // With off32: the K scatters as 32-bit byte offsets, entries from K up to a
+  // multiple of 64 holding the out-of-range sentinel 0x40000000.
+  const uint32_t* ak32;
+  const uint32_t* bk32;
 };
 
 // One item of a grouped launch: the contraction plus where its tiles start in

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstring>
 #include <memory>
 
 #include "kernels.h"
@@ -253,6 +254,17 @@ void configure(Plan* plan, const PlanConfig& cfg) {
     return *rlo >= 0 && *klo >= 0 && (__int128)(*rhi + *khi + 1) * es <= (__int128)(1 << 30);
   };
   p.off32 = fits32(p.am, p.ak) && fits32(p.bn, p.bk);
+  p.ak32.clear();
+  p.bk32.clear();
+  if (p.off32) {
+    const size_t padded = (size_t)((p.K + 63) / 64 * 64);
+    p.ak32.assign(padded, 0x40000000u);
+    p.bk32.assign(padded, 0x40000000u);
+    for (int64_t k = 0; k < p.K; k++) {
+      p.ak32[k] = (uint32_t)(p.ak[k] * es);
+      p.bk32[k] = (uint32_t)(p.bk[k] * es);
+    }
+  }
 }
 
 Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
@@ -292,15 +304,21 @@ const DeviceScatter& Plan::on_device(int device) {
   std::vector<const std::vector<int64_t>*> parts = {&am, &ak, &bk, &bn, &cm, &cn, &dm, &dn, &ha, &hb, &hc, &hd, &dr};
   size_t total = 0;
   for (auto* v : parts) total += v->size();
+  // The 32-bit K offsets follow, two per 64-bit word (each array starts on a word).
+  const size_t w32a = (ak32.size() + 1) / 2, w32b = (bk32.size() + 1) / 2;
   std::vector<int64_t> flat;
-  flat.reserve(total);
+  flat.reserve(total + w32a + w32b);
   for (auto* v : parts) flat.insert(flat.end(), v->begin(), v->end());
+  flat.resize(total + w32a + w32b, 0);
+  if (!ak32.empty()) std::memcpy(flat.data() + total, ak32.data(), ak32.size() * 4);
+  if (!bk32.empty()) std::memcpy(flat.data() + total + w32a, bk32.data(), bk32.size() * 4);
+  const size_t words = flat.size();
   int prev = 0;
   (void)hipGetDevice(&prev);
   if (prev != device) (void)hipSetDevice(device);
   DeviceScatter ds{};
-  hipError_t e = hipMalloc(&ds.base, total * sizeof(int64_t));
-  if (e == hipSuccess) e = hipMemcpy(ds.base, flat.data(), total * sizeof(int64_t), hipMemcpyHostToDevice);
+  hipError_t e = hipMalloc(&ds.base, words * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMemcpy(ds.base, flat.data(), words * sizeof(int64_t), hipMemcpyHostToDevice);
   if (prev != device) (void)hipSetDevice(prev);
   if (e != hipSuccess) {
     if (ds.base) (void)hipFree(ds.base);
@@ -313,6 +331,8 @@ const DeviceScatter& Plan::on_device(int device) {
     *slots[i] = q;
     q += parts[i]->size();
   }
+  ds.ak32 = ak32.empty() ? nullptr : (const uint32_t*)q;
+  ds.bk32 = bk32.empty() ? nullptr : (const uint32_t*)(q + w32a);
   return dev.emplace(device, ds).first->second;
 }
 
@@ -357,6 +377,8 @@ ContractArgs Plan::args(int device, const void* A, const void* B, const void* C,
   a.a_kvec = a_kvec && (uintptr_t)A % 16 == 0;
   a.b_kvec = b_kvec && (uintptr_t)B % 16 == 0;
   a.off32 = off32;
+  a.ak32 = s.ak32;
+  a.bk32 = s.bk32;
   return a;
 }
 
@@ -373,6 +395,7 @@ std::string Plan::kernel_name() const {
   }
   const TileCfg* t = tile_cfg(tile);
   if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(t->bk);
+  if (cfg.precision == PREC_STORAGE && dma_supported(off32, prob.dtype, cfg.complex_method, tile)) s += ".dma";
   return s;
 }
 

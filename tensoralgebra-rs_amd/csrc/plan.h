@@ -32,6 +32,7 @@ struct PlanConfig {
 struct DeviceScatter {
   int64_t* base = nullptr;
   const int64_t *am, *ak, *bk, *bn, *cm, *cn, *dm, *dn, *ha, *hb, *hc, *hd, *dr;
+  const uint32_t *ak32, *bk32;
 };
 
 struct Plan {
@@ -45,6 +46,9 @@ struct Plan {
   // checked per call).
   bool a_kvec = false, b_kvec = false;
   bool off32 = false;  // ContractArgs::off32
+  // With off32: the K scatters of A and B as byte offsets, padded with the
+  // out-of-range sentinel to a multiple of 64 entries (ContractArgs::ak32).
+  std::vector<uint32_t> ak32, bk32;
   int tile = 0;
   int strategy = 0;  // 0 contraction, 1 output pass only (K empty), 2 output empty
 
