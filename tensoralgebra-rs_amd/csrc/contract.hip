@@ -85,7 +85,13 @@ struct Cfg {
       METHOD == M_REAL || METHOD == M_1M ? 1 : (METHOD == M_NATIVE && NEG_MFMA ? 2 : 3);
   static constexpr int STAGE = PLANES * (A_PLANE + B_PLANE);  // T values of one LDS stage
   static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * sizeof(T);
-  static constexpr size_t LDS_EPI = (size_t)(CPLX ? 2 : 1) * BM * LDD * sizeof(T);
+  // The finished tile is staged in row chunks: the whole tile when it fits in
+  // the main loop's LDS, otherwise one wave row (WM rows) at a time, so the
+  // staging never raises the workgroup's LDS footprint above the main loop's
+  // by more than one wave row.
+  static constexpr size_t EPI_ROW_BYTES = (size_t)(CPLX ? 2 : 1) * LDD * sizeof(T);
+  static constexpr int EPI_ROWS = (size_t)BM * EPI_ROW_BYTES <= LDS_MAIN ? BM : WM;
+  static constexpr size_t LDS_EPI = (size_t)EPI_ROWS * EPI_ROW_BYTES;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
   __device__ static inline int aidx(int m, int k) { return AKM ? m * LDAK + k : k * LDAM + m; }
   __device__ static inline int bidx(int n, int k) { return BKM ? n * LDBK + k : k * LDBN + n; }
@@ -379,9 +385,15 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
 #undef TP_LOAD_KOFF
 #undef TP_GATHER
 
-  // Stage the tile: Dre/Dim [BM][LDD].
+  // Stage the tile chunk by chunk (Dre/Dim [EPI_ROWS][LDD]) and write it back.
+  constexpr int ER = C::EPI_ROWS;
   T* Dre = (T*)smem;
-  T* Dim = Dre + BM * C::LDD;
+  T* Dim = Dre + ER * C::LDD;
+#pragma unroll 1
+  for (int c = 0; c < BM / ER; c++) {
+    if (c > 0) __syncthreads();  // the previous chunk's write-back has read the staging rows
+    if (wm / ER == c) {
+      const int wr = wm - c * ER;  // this wave's first row inside the chunk
 #pragma unroll
   for (int i = 0; i < MB; i++)
 #pragma unroll
@@ -392,11 +404,11 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
         const int rr = MM::row(lk, q);  // row inside the 16x16 block
         if constexpr (C::METH == M_1M) {
           // real row rr of the 1e product is complex row rr / 2, component rr % 2
-          const int row = wm + 8 * i + (rr >> 1);
+          const int row = wr + 8 * i + (rr >> 1);
           if (rr & 1) Dim[row * C::LDD + col] = acc[0][i][j][q];
           else Dre[row * C::LDD + col] = acc[0][i][j][q];
         } else {
-          const int row = wm + 16 * i + rr;
+          const int row = wr + 16 * i + rr;
           if constexpr (C::METH == M_REAL) {
             Dre[row * C::LDD + col] = acc[0][i][j][q];
           } else if constexpr (C::METH == M_NATIVE && C::NEG_MFMA) {
@@ -412,8 +424,10 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
           }
         }
       }
-  __syncthreads();
-  dev::write_back<T, T, C::CPLX, BM, BN, NT, C::LDD>(p, h, m0, n0, Dre, Dim);
+    }
+    __syncthreads();
+    dev::write_back<T, T, C::CPLX, ER, BN, NT, C::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
+  }
 }
 
 template <class C>

@@ -253,6 +253,10 @@ void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in
   ContractArgs a = p.args(ex.device, wa.dev, wb.dev, wc.dev, const_cast<void*>(wd.dev), alpha, beta, read_c);
   if (!product) {
     hip_check(launch_output_pass(a, p.prob.dtype, s), "output-pass launch");
+  } else if (p.cfg.precision == PREC_FP8) {
+    std::lock_guard<std::mutex> g(ex.panel_mu);
+    char* panels = ex.ensure_panels(fp8_panel_bytes(a, p.prob.dtype));
+    hip_check(launch_contract_fp8(a, p.prob.dtype, panels, s), "FP8 contraction launch");
   } else if (p.cfg.precision == PREC_SPLIT_BF16_2 || p.cfg.precision == PREC_SPLIT_BF16_3) {
     const int terms = p.cfg.precision == PREC_SPLIT_BF16_2 ? 2 : 3;
     std::lock_guard<std::mutex> g(ex.panel_mu);

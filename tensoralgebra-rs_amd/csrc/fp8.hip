@@ -1,0 +1,366 @@
+// FP8 precision for f32/c32 storage: operands quantised to OCP e4m3 with one
+// power-of-two scale per 128-element K panel of every row, multiplied on
+// gfx950's scaled fp8 MFMA (v_mfma_scale_f32_16x16x128_f8f6f4, fp32
+// accumulation), which applies both operands' scales in the instruction. No
+// counterpart exists in the reference (crates/tprims-capi/src/product.rs:132
+// refuses any precision other than the storage one); gather, conjugation and
+// write-back semantics are those of the storage kernels.
+//
+// Scale of a panel (row r, K block kb of 128): e = the smallest integer with
+// amax * 2^-e <= 448 (the largest e4m3 value), stored as the E8M0 exponent
+// 127 + e; elements are converted as x * 2^-e with round-to-nearest. A complex
+// operand has separate re and im panels, each with its own scales.
+//
+// Instruction layout (checked on gfx950 with known inputs): lane l supplies
+// row l % 16 and 32 bytes of fp8; A and B lanes pair byte for byte, and the
+// lane's scale register (its low byte) scales one 32-k quarter of the 128-k
+// step. With one scale per 128-k panel every quarter of a row carries the
+// same value, so the layout of k inside the 32 bytes does not matter; the
+// packer gives lane l the k range 32 (l / 16) .. + 31 of its row.
+//
+// Two stages, as in the split-BF16 path:
+//  1. k_pack_fp8: one wave per (batch h, 128-k block, 16-row block): each lane
+//     gathers its 32 values, the four lanes of a row reduce the panel's amax
+//     with cross-lane shuffles, and the lane stores its 32 bytes of the 2 KiB
+//     fragment plus (lanes of the first quarter) the row's int32 scale word.
+//  2. k_gemm_fp8: per 128-k step the workgroup copies its row blocks'
+//     fragments and scale words into LDS by DMA; each wave reads 32 bytes per
+//     fragment and runs one scaled MFMA per 16x16 block and real product.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace tp {
+namespace {
+
+using dev::Store;
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+constexpr int F8_FRAG = 2048;  // bytes per fragment (16 rows x 128 k)
+constexpr int F8_KBLK = 128;
+constexpr float F8_MAX = 448.0f;
+
+struct F8PackArgs {
+  const void* X;
+  const int64_t* rv;  // row scatter (length R)
+  const int64_t* kv;  // K scatter (length K)
+  const int64_t* hv;  // batch scatter (length H)
+  int64_t R, K, H;
+  int64_t RB;  // row blocks (of 16) per (h, K block), padded to whole tiles and to 4
+  int64_t KB;  // K blocks of 128
+  int32_t conj;
+  int32_t vec;  // ContractArgs::a_kvec / b_kvec of this operand
+  uint8_t* out;
+  int32_t* scales;  // [h][kb][RB * 16 rows][comp]
+};
+
+// Exponent e of the panel scale 2^e for a panel maximum amax (0 for an all-zero
+// panel): the smallest e with amax * 2^-e <= 448.
+__device__ inline int panel_exponent(float amax) {
+  if (!(amax > 0.0f)) return 0;
+  int E;
+  (void)frexpf(amax / F8_MAX, &E);  // amax / 448 in [2^(E-1), 2^E)
+  int e = E - 1;
+  if (ldexpf(amax, -e) > F8_MAX) e = E;
+  return e < -126 ? -126 : (e > 127 ? 127 : e);
+}
+
+template <bool CPLX, bool VEC>
+__global__ __launch_bounds__(256) void k_pack_fp8(F8PackArgs p) {
+  typedef Store<float, CPLX> St;
+  typedef typename St::S S;
+  constexpr int COMPS = CPLX ? 2 : 1;
+  const int lane = threadIdx.x & 63;
+  const int64_t rgroups = p.RB / 4;
+  int64_t bid = blockIdx.x;
+  const int64_t rg = bid % rgroups;
+  bid /= rgroups;
+  const int64_t kb = bid % p.KB;
+  const int64_t h = bid / p.KB;
+  const int64_t rb = rg * 4 + (threadIdx.x >> 6);
+  const int64_t r = rb * 16 + (lane & 15);
+  const int64_t k0 = kb * F8_KBLK + 32 * (lane >> 4);
+  const bool rok = r < p.R;
+  // Row 0 and k 0 have offset 0: X (and X + 0..7 under VEC) is a valid address.
+  const S* __restrict__ X = (const S*)p.X + p.hv[h] + (rok ? p.rv[r] : 0);
+  float re[32], im[32];
+#pragma unroll
+  for (int c8 = 0; c8 < 4; c8++) {
+    if constexpr (VEC) {
+      const bool ok = rok && k0 + 8 * c8 < p.K;  // K % 8 == 0: 8 k in or out together
+      const v4f* src = (const v4f*)(X + (ok ? k0 + 8 * c8 : 0));
+      if constexpr (!CPLX) {
+        const v4f x0 = src[0], x1 = src[1];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          re[8 * c8 + j] = ok ? x0[j] : 0.0f;
+          re[8 * c8 + 4 + j] = ok ? x1[j] : 0.0f;
+          im[8 * c8 + j] = im[8 * c8 + 4 + j] = 0.0f;
+        }
+      } else {
+        v4f x[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) x[q] = src[q];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          re[8 * c8 + j] = ok ? x[j >> 1][2 * (j & 1)] : 0.0f;
+          im[8 * c8 + j] = ok ? x[j >> 1][2 * (j & 1) + 1] : 0.0f;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int64_t k = k0 + 8 * c8 + j;
+        const bool ok = rok && k < p.K;
+        const S v = X[ok ? p.kv[k] : 0];
+        re[8 * c8 + j] = ok ? St::re(v) : 0.0f;
+        im[8 * c8 + j] = ok ? St::im(v) : 0.0f;
+      }
+    }
+  }
+  uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rb) * COMPS * F8_FRAG + lane * 32;
+#pragma unroll
+  for (int c = 0; c < COMPS; c++) {
+    const bool neg = c == 1 && p.conj;
+    float amax = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 32; j++) amax = fmaxf(amax, fabsf(c == 0 ? re[j] : im[j]));
+    amax = fmaxf(amax, __shfl_xor(amax, 16));
+    amax = fmaxf(amax, __shfl_xor(amax, 32));
+    const int e = panel_exponent(amax);
+    const float inv = ldexpf(neg ? -1.0f : 1.0f, -e);
+    int w[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+      const float* v = (c == 0 ? re : im) + 4 * d;
+      int x = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, 0, false);
+      w[d] = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, x, true);
+    }
+    v4u* dst = (v4u*)(out + c * F8_FRAG);
+    dst[0] = v4u{(unsigned)w[0], (unsigned)w[1], (unsigned)w[2], (unsigned)w[3]};
+    dst[1] = v4u{(unsigned)w[4], (unsigned)w[5], (unsigned)w[6], (unsigned)w[7]};
+    if (lane < 16) p.scales[((h * p.KB + kb) * p.RB * 16 + r) * COMPS + c] = 127 + e;
+  }
+}
+
+template <bool CPLX_, int BM_, int BN_, int WM_, int WN_>
+struct F8Cfg {
+  static constexpr bool CPLX = CPLX_;
+  static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;
+  static constexpr int WAVES_M = BM / WM;
+  static constexpr int NW = (BM / WM) * (BN / WN);
+  static constexpr int NT = NW * 64;
+  static constexpr int COMPS = CPLX ? 2 : 1;
+  static constexpr int A_FRAGS = BM / 16 * COMPS, B_FRAGS = BN / 16 * COMPS;
+  // One stage: fragments (2 KiB each, copied as two 1 KiB DMA pieces), then
+  // the scale words of A's and B's rows (int32 [row][comp], 256 B per piece).
+  static constexpr int PIECES = 2 * (A_FRAGS + B_FRAGS);
+  static constexpr int A_SC = BM * COMPS * 4, B_SC = BN * COMPS * 4;  // bytes
+  static constexpr int SC_PIECES = (A_SC + B_SC) / 256;
+  static constexpr size_t FRAG_BYTES = (size_t)(A_FRAGS + B_FRAGS) * F8_FRAG;
+  static constexpr size_t STAGE_BYTES = FRAG_BYTES + A_SC + B_SC;
+  static constexpr int MI = WM / 16, NI = WN / 16;
+  static constexpr int LDD = BN + 1;
+  static constexpr size_t LDS_MAIN = 2 * STAGE_BYTES;
+  static constexpr size_t LDS_EPI = (size_t)COMPS * BM * LDD * 4;
+  static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
+  static_assert(PIECES % NW == 0, "whole fragment pieces per wave");
+  static_assert(A_SC % 256 == 0 && B_SC % 256 == 0, "scale pieces");
+  static_assert(LDS <= 160 * 1024, "LDS budget");
+};
+
+struct F8GemmArgs {
+  ContractArgs c;
+  const uint8_t* Ap;
+  const uint8_t* Bp;
+  const int32_t* As;  // scales
+  const int32_t* Bs;
+  int64_t RBa, RBb, KB;
+};
+
+template <class G>
+__global__ __launch_bounds__(G::NT) void k_gemm_fp8(F8GemmArgs g) {
+  const ContractArgs& p = g.c;
+  __shared__ __attribute__((aligned(16))) char smem[G::LDS];
+  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
+  int64_t h;
+  int tm, tn;
+  dev::decode_tile(blockIdx.x, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
+  constexpr int BM = G::BM, BN = G::BN, COMPS = G::COMPS;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = (wave % G::WAVES_M) * G::WM;
+  const int wn = (wave / G::WAVES_M) * G::WN;
+
+  // Fragments of K block kb: A's BM/16 row blocks (all comps) contiguous in the
+  // panel, then B's; the scale words of the tile's rows likewise.
+  const uint8_t* Ag = g.Ap + (h * g.KB * g.RBa + (int64_t)tm * (BM / 16)) * COMPS * F8_FRAG;
+  const uint8_t* Bg = g.Bp + (h * g.KB * g.RBb + (int64_t)tn * (BN / 16)) * COMPS * F8_FRAG;
+  const int32_t* Asg = g.As + (h * g.KB * g.RBa * 16 + (int64_t)tm * BM) * COMPS;
+  const int32_t* Bsg = g.Bs + (h * g.KB * g.RBb * 16 + (int64_t)tn * BN) * COMPS;
+  const int64_t a_step = g.RBa * COMPS * F8_FRAG, b_step = g.RBb * COMPS * F8_FRAG;
+  const int64_t as_step = g.RBa * 16 * COMPS, bs_step = g.RBb * 16 * COMPS;
+  constexpr int PW = G::PIECES / G::NW;
+  auto issue = [&](int64_t kb, int st) {
+    char* stage = smem + (size_t)st * G::STAGE_BYTES;
+#pragma unroll
+    for (int i = 0; i < PW; i++) {
+      const int f = wave + i * G::NW;  // 1 KiB piece index
+      const uint8_t* src = f < 2 * G::A_FRAGS ? Ag + kb * a_step + (int64_t)f * 1024
+                                              : Bg + kb * b_step + (int64_t)(f - 2 * G::A_FRAGS) * 1024;
+      __builtin_amdgcn_global_load_lds((const void*)(src + lane * 16),
+                                       (__attribute__((address_space(3))) void*)(stage + (size_t)f * 1024), 16, 0, 0);
+    }
+    for (int f = wave; f < G::SC_PIECES; f += G::NW) {  // 256 B scale pieces
+      const int32_t* src = f < G::A_SC / 256 ? Asg + kb * as_step + f * 64 : Bsg + kb * bs_step + (f - G::A_SC / 256) * 64;
+      __builtin_amdgcn_global_load_lds((const void*)(src + lane),
+                                       (__attribute__((address_space(3))) void*)(stage + G::FRAG_BYTES + (size_t)f * 256),
+                                       4, 0, 0);
+    }
+  };
+
+  constexpr int ACCS = G::CPLX ? 3 : 1;
+  v4f acc[ACCS][G::MI][G::NI];
+#pragma unroll
+  for (int s = 0; s < ACCS; s++)
+#pragma unroll
+    for (int i = 0; i < G::MI; i++)
+#pragma unroll
+      for (int j = 0; j < G::NI; j++) acc[s][i][j] = v4f{0, 0, 0, 0};
+
+  const int64_t nk = g.KB;
+  issue(0, 0);
+  if (nk > 1) issue(1, 1);
+  __builtin_amdgcn_s_waitcnt(7 << 4);  // vmcnt(0) lgkmcnt(0)
+  __syncthreads();
+  const int lr = lane & 15;
+  for (int64_t kb = 0; kb < nk; kb++) {
+    const char* stage = smem + (size_t)(kb & 1) * G::STAGE_BYTES;
+    const int32_t* sc = (const int32_t*)(stage + G::FRAG_BYTES);
+    v8i a[COMPS][G::MI], b[COMPS][G::NI];
+    int sa[COMPS][G::MI], sb[COMPS][G::NI];
+#pragma unroll
+    for (int c = 0; c < COMPS; c++) {
+#pragma unroll
+      for (int i = 0; i < G::MI; i++) {
+        const v4u* q = (const v4u*)(stage + (size_t)((((wm >> 4) + i) * COMPS + c) * F8_FRAG) + lane * 32);
+        const v4u lo = q[0], hi = q[1];
+        a[c][i] = v8i{(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3], (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
+        sa[c][i] = sc[(wm + 16 * i + lr) * COMPS + c];
+      }
+#pragma unroll
+      for (int j = 0; j < G::NI; j++) {
+        const v4u* q = (const v4u*)(stage + (size_t)((G::A_FRAGS + ((wn >> 4) + j) * COMPS + c) * F8_FRAG) + lane * 32);
+        const v4u lo = q[0], hi = q[1];
+        b[c][j] = v8i{(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3], (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
+        sb[c][j] = sc[BM * COMPS + (wn + 16 * j + lr) * COMPS + c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < COMPS; c++)
+#pragma unroll
+      for (int d = 0; d < COMPS; d++) {
+        const int slot = !G::CPLX ? 0 : (c == 0 && d == 0 ? 0 : (c == 1 && d == 1 ? 1 : 2));
+#pragma unroll
+        for (int i = 0; i < G::MI; i++)
+#pragma unroll
+          for (int j = 0; j < G::NI; j++)
+            acc[slot][i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[c][i], b[d][j], acc[slot][i][j], 0, 0, 0,
+                                                                               sa[c][i], 0, sb[d][j]);
+      }
+    // Own pieces of block kb + 1 landed and reads of block kb done; after the
+    // barrier block kb + 1 is visible and stage kb & 1 takes block kb + 2.
+    __builtin_amdgcn_s_waitcnt(7 << 4);
+    __syncthreads();
+    if (kb + 2 < nk) issue(kb + 2, (int)(kb & 1));
+  }
+
+  float* Dre = (float*)smem;
+  float* Dim = Dre + BM * G::LDD;
+  const int lk = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < G::MI; i++)
+#pragma unroll
+    for (int j = 0; j < G::NI; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = wm + 16 * i + 4 * lk + q, col = wn + 16 * j + lr;
+        if constexpr (G::CPLX) {
+          Dre[row * G::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
+          Dim[row * G::LDD + col] = acc[2][i][j][q];
+        } else {
+          Dre[row * G::LDD + col] = acc[0][i][j][q];
+        }
+      }
+  __syncthreads();
+  dev::write_back<float, float, G::CPLX, BM, BN, G::NT, G::LDD>(p, h, (int64_t)tm * BM, (int64_t)tn * BN, Dre, Dim);
+}
+
+struct Shape {
+  int bm, bn;
+};
+
+Shape f8_shape(bool cplx) { return cplx ? Shape{128, 64} : Shape{256, 128}; }
+
+int64_t pad_rb(int64_t rows, int bm) {
+  const int64_t rb = (rows + bm - 1) / bm * (bm / 16);
+  return (rb + 3) / 4 * 4;
+}
+
+template <bool CPLX>
+hipError_t run(const ContractArgs& a, void* panels, hipStream_t s) {
+  constexpr int COMPS = CPLX ? 2 : 1;
+  const Shape sh = f8_shape(CPLX);
+  const int64_t KB = (a.K + F8_KBLK - 1) / F8_KBLK;
+  const int64_t RBa = pad_rb(a.M, sh.bm), RBb = pad_rb(a.N, sh.bn);
+  uint8_t* Ap = (uint8_t*)panels;
+  uint8_t* Bp = Ap + a.H * KB * RBa * COMPS * F8_FRAG;
+  int32_t* As = (int32_t*)(Bp + a.H * KB * RBb * COMPS * F8_FRAG);
+  int32_t* Bs = As + a.H * KB * RBa * 16 * COMPS;
+  F8PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kvec, Ap, As};
+  F8PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, Bp, Bs};
+  for (const F8PackArgs* q : {&pa, &pb}) {
+    const int64_t grid = q->H * q->KB * (q->RB / 4);
+    if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
+    if (q->vec) hipLaunchKernelGGL((k_pack_fp8<CPLX, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
+    else hipLaunchKernelGGL((k_pack_fp8<CPLX, false>), dim3((unsigned)grid), dim3(256), 0, s, *q);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  F8GemmArgs g{a, Ap, Bp, As, Bs, RBa, RBb, KB};
+  g.c.tiles_m = (int)((a.M + sh.bm - 1) / sh.bm);
+  g.c.tiles_n = (int)((a.N + sh.bn - 1) / sh.bn);
+  const int64_t nwg = (int64_t)g.c.tiles_m * g.c.tiles_n * a.H;
+  if (nwg > 0x7fffffff) return hipErrorInvalidConfiguration;
+  if constexpr (CPLX) {
+    typedef F8Cfg<true, 128, 64, 32, 32> G;
+    hipLaunchKernelGGL(k_gemm_fp8<G>, dim3((unsigned)nwg), dim3(G::NT), 0, s, g);
+  } else {
+    typedef F8Cfg<false, 256, 128, 64, 64> G;
+    hipLaunchKernelGGL(k_gemm_fp8<G>, dim3((unsigned)nwg), dim3(G::NT), 0, s, g);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t fp8_panel_bytes(const ContractArgs& a, DType t) {
+  const bool cplx = t == DType::C32;
+  const Shape sh = f8_shape(cplx);
+  const int64_t KB = (a.K + F8_KBLK - 1) / F8_KBLK;
+  const int64_t rows = pad_rb(a.M, sh.bm) + pad_rb(a.N, sh.bn);  // row blocks
+  const int64_t comps = cplx ? 2 : 1;
+  return (size_t)(a.H * KB * rows * comps * (F8_FRAG + 16 * 4));
+}
+
+hipError_t launch_contract_fp8(const ContractArgs& a, DType t, void* panels, hipStream_t s) {
+  if (t == DType::F32) return run<false>(a, panels, s);
+  if (t == DType::C32) return run<true>(a, panels, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace tp

@@ -43,6 +43,12 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int tile_id, hipS
 size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id);
 hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int tile_id, void* panels,
                                  hipStream_t s);
+// FP8 (e4m3, one power-of-two scale per 128-k panel of every row) for f32/c32
+// storage: packs A and B into fp8 fragment panels plus scale words (device
+// scratch of fp8_panel_bytes), then multiplies on the scaled fp8 MFMA. Tiles:
+// 256x128 (f32), 128x64 (c32).
+size_t fp8_panel_bytes(const ContractArgs& a, DType t);
+hipError_t launch_contract_fp8(const ContractArgs& a, DType t, void* panels, hipStream_t s);
 // D = op_D(beta * op_C(C)) over the output (empty K or alpha == 0).
 hipError_t launch_output_pass(const ContractArgs& a, DType t, hipStream_t s);
 // Zero every element a layout addresses (rank <= 8; used by the adjoints).

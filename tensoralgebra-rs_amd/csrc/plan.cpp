@@ -113,6 +113,7 @@ Axis scaled(const Axis& a, int64_t extent, int64_t by) {
 }
 
 bool tile_allowed(int id, DType t, int prec) {
+  if (prec == PREC_FP8) return id == (is_complex(t) ? STILE_128x64 : STILE_256x128);
   if (prec != PREC_STORAGE)
     return id == STILE_64x64 || (id == STILE_128x64 && is_complex(t)) ||
            ((id == STILE_128x128 || id == STILE_256x128) && !is_complex(t));
@@ -128,7 +129,9 @@ int choose_tile(const Plan& p) {
     return p.cfg.tile;
   }
   std::vector<std::pair<int, double>> cands;
-  if (prec != PREC_STORAGE) {
+  if (prec == PREC_FP8) {
+    return is_complex(p.prob.dtype) ? STILE_128x64 : STILE_256x128;
+  } else if (prec != PREC_STORAGE) {
     cands = {{STILE_256x128, 1.3}, {STILE_128x128, 1.15}, {STILE_128x64, 1.2}, {STILE_64x64, 1.0}};
   } else {
     cands = {{TILE_128x64, 1.1}, {TILE_64x64, 1.0}, {TILE_32x32, 0.7}};
@@ -157,11 +160,10 @@ void configure(Plan* plan, const PlanConfig& cfg) {
   p.cfg = cfg;
   if (cfg.precision != PREC_STORAGE && is_double(pr.dtype))
     fail(ERR_UNSUPPORTED, "split-BF16 / FP8 precision applies to f32 and c32 only");
-  if (cfg.precision == PREC_FP8) fail(ERR_UNSUPPORTED, "the FP8 path is not available in this build");
   if (cfg.complex_method < CPLX_NATIVE || cfg.complex_method > CPLX_3M)
     fail(ERR_INVALID_ARGUMENT, "unknown complex method");
   if (cfg.precision != PREC_STORAGE && cfg.complex_method != CPLX_NATIVE)
-    fail(ERR_UNSUPPORTED, "split-BF16 runs the native complex schedule only");
+    fail(ERR_UNSUPPORTED, "split-BF16 and FP8 run the native complex schedule only");
 
   p.tile = choose_tile(p);
   const TileCfg* t = tile_cfg(p.tile);
@@ -388,13 +390,14 @@ std::string Plan::kernel_name() const {
   std::string s = dtype_name(prob.dtype);
   if (cfg.precision == PREC_SPLIT_BF16_2) s += ".split_bf16x2";
   else if (cfg.precision == PREC_SPLIT_BF16_3) s += ".split_bf16x3";
+  else if (cfg.precision == PREC_FP8) s += ".fp8_e4m3";
   else s += is_double(prob.dtype) ? ".mfma_f64" : ".mfma_f32";
   if (is_complex(prob.dtype)) {
     static const char* m[3] = {".native", ".1m", ".3m"};
     s += m[cfg.complex_method];
   }
   const TileCfg* t = tile_cfg(tile);
-  if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(t->bk);
+  if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(cfg.precision == PREC_FP8 ? 128 : t->bk);
   if (cfg.precision == PREC_STORAGE && dma_supported(off32, prob.dtype, cfg.complex_method, tile)) s += ".dma";
   return s;
 }
