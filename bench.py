@@ -29,7 +29,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "tensoralgebra-rs_amd")]
 
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2516.6}
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2516.6, "fp8": 5033.2}
 
 
 def parse():
@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--dtype", default="c64", choices=["c64", "c32"])
     ap.add_argument("--method", default="native", choices=["native", "1m", "3m"])
-    ap.add_argument("--precision", default="storage", choices=["storage", "split2", "split3"])
+    ap.add_argument("--precision", default="storage", choices=["storage", "split2", "split3", "fp8"])
     ap.add_argument("--only", default=None, help="run only this part: main, variants, configs, cpu")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
@@ -93,7 +93,7 @@ class Complex:
                                       tp.Labels([J, I, L, N], [L, K, J, N], [I, K, N], [I, K, N]))
         cm = {"native": tp.ComplexMethod.NATIVE, "1m": tp.ComplexMethod.ONE_M, "3m": tp.ComplexMethod.THREE_M}[method]
         pr = {"storage": tp.Precision.STORAGE, "split2": tp.Precision.SPLIT_BF16_2,
-              "split3": tp.Precision.SPLIT_BF16_3}[precision]
+              "split3": tp.Precision.SPLIT_BF16_3, "fp8": tp.Precision.FP8_E4M3}[precision]
         self.plan = tp.Plan(prob, tp.PlanConfig(complex_method=cm, precision=pr))
         self.flops = 8 * i * k * j * l * n
         es = 16 if dtype == "c64" else 8
@@ -181,7 +181,7 @@ def config_precision(torch, tp, np, ex, stream, steps, shape):
     want = np.array([Ah[:, i] @ Bh[j, :] for i, j in zip(si, sj)])
     del A64, B64
     for name, prec in (("fp32", tp.Precision.STORAGE), ("split_bf16x3", tp.Precision.SPLIT_BF16_3),
-                       ("split_bf16x2", tp.Precision.SPLIT_BF16_2)):
+                       ("split_bf16x2", tp.Precision.SPLIT_BF16_2), ("fp8_e4m3", tp.Precision.FP8_E4M3)):
         prob = tp.Problem.from_labels(tp.DType.F32, specs(a_dims), specs(b_dims), tp.CSpec.separate(specs(d_dims)),
                                       specs(d_dims), tp.Labels(a_lab, b_lab, d_lab, d_lab))
         plan = tp.Plan(prob, tp.PlanConfig(precision=prec))
@@ -190,8 +190,8 @@ def config_precision(torch, tp, np, ex, stream, steps, shape):
         Dh = D.cpu().numpy().reshape(N, M)
         got = np.array([Dh[j, i] for i, j in zip(si, sj)], dtype=np.float64)
         tf = out["flops"] / ms / 1e9
-        peak = PEAK_TFLOPS["f32"] if prec == tp.Precision.STORAGE else \
-            PEAK_TFLOPS["bf16"] / (6 if prec == tp.Precision.SPLIT_BF16_3 else 3)
+        peak = {tp.Precision.STORAGE: PEAK_TFLOPS["f32"], tp.Precision.SPLIT_BF16_3: PEAK_TFLOPS["bf16"] / 6,
+                tp.Precision.SPLIT_BF16_2: PEAK_TFLOPS["bf16"] / 3, tp.Precision.FP8_E4M3: PEAK_TFLOPS["fp8"]}[prec]
         out[name] = {"ms": round(ms, 4), "tflops": round(tf, 3), "frac_of_derived_peak": round(tf / peak, 4),
                      "rel_err_vs_fp64": rel_err(np, got, want), "kernel": plan.report()["kernel"]}
     out["split3_vs_fp32"] = round(out["fp32"]["ms"] / out["split_bf16x3"]["ms"], 3)
@@ -305,6 +305,8 @@ def main():
         peak = PEAK_TFLOPS["bf16"] / 6
     elif args.precision == "split2":
         peak = PEAK_TFLOPS["bf16"] / 3
+    elif args.precision == "fp8":
+        peak = PEAK_TFLOPS["fp8"]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_complex_benchmark.json")
     if os.path.exists(tpath):
@@ -336,6 +338,7 @@ def main():
     if not args.no_variants:
         variants = {}
         for dt, m, p in [("c32", "native", "storage"), ("c32", "native", "split3"), ("c32", "native", "split2"),
+                         ("c32", "native", "fp8"),
                          ("c64", "1m", "storage"), ("c64", "3m", "storage"), ("c32", "3m", "storage")]:
             if (dt, m, p) == (args.dtype, args.method, args.precision):
                 continue

@@ -84,7 +84,7 @@ typedef struct {
   int32_t a_k_major, b_k_major; /* operand gathers run along K (1) or along M/N (0) */
   int32_t d_m_major;        /* write-back runs along M (1) or N (0) */
   int32_t m_axes, n_axes, k_axes, h_axes; /* folded axis counts */
-  int32_t strategy;         /* 0 contraction kernel, 1 output pass only, 2 empty */
+  int32_t strategy;         /* 0 contraction kernel, 1 output pass only, 2 empty, 3 elementwise */
   char kernel[64];
 } tprims_plan_report;
 TAPP_EXPORT TAPP_error tprims_product_report(TAPP_tensor_product plan, tprims_plan_report* out);

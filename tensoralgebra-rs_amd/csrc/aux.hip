@@ -1,7 +1,9 @@
-// Output-only kernels: the output pass D = op_D(beta * op_C(C)) used for an
-// empty contraction or alpha == 0 (crates/tprims-kernel/src/pack/writeback.rs:495
-// scale_only; crates/tprims-contract/src/strategy/elementwise.rs:210), and the
-// zero fill of a layout used by the adjoints. Both are HBM-bound element passes.
+// Element-parallel kernels: the output pass D = op_D(beta * op_C(C)) used for
+// an empty contraction or alpha == 0 (crates/tprims-kernel/src/pack/writeback.rs:495
+// scale_only; crates/tprims-contract/src/strategy/elementwise.rs:210), the
+// elementwise (all-batch) strategy for tiny M x N and K with many batch entries
+// (strategy/elementwise.rs:194, the Hadamard-type products), and the zero fill
+// of a layout used by the adjoints. All are HBM-bound element passes.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -52,6 +54,79 @@ __global__ void k_output_pass(ContractArgs p) {
   }
 }
 
+// One thread per output element (h, m, n), K summed serially in the storage
+// type, then the output-pass write-back. Elements run along D's contiguous
+// role: the batch index when M = N = 1, else M or N as the write-back does.
+template <typename T, bool CPLX>
+__global__ void k_elementwise(ContractArgs p) {
+  typedef Store<T, CPLX> St;
+  typedef typename St::S S;
+  const int64_t M = p.M, N = p.N, H = p.H, K = p.K;
+  const int64_t total = M * N * H;
+  const T alr = (T)p.alpha[0], ali = (T)p.alpha[1];
+  const T ber = (T)p.beta[0], bei = (T)p.beta[1];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t m, n, h;
+    if (M * N == 1) {
+      m = n = 0;
+      h = e;
+    } else if (p.d_mmajor) {
+      m = e % M;
+      const int64_t r = e / M;
+      n = r % N;
+      h = r / N;
+    } else {
+      n = e % N;
+      const int64_t r = e / N;
+      m = r % M;
+      h = r / M;
+    }
+    int64_t oa, ob, oc, od;
+    if (p.h_affine) {
+      oa = h * p.h_stride[0];
+      ob = h * p.h_stride[1];
+      oc = h * p.h_stride[2];
+      od = h * p.h_stride[3];
+    } else {
+      oa = p.ha[h];
+      ob = p.hb[h];
+      oc = p.read_c ? p.hc[h] : 0;
+      od = p.hd[h];
+    }
+    const S* A = (const S*)p.A + oa + p.am[m];
+    const S* B = (const S*)p.B + ob + p.bn[n];
+    T sr = 0, si = 0;
+    for (int64_t k = 0; k < K; k++) {
+      const S av = A[p.ak[k]], bv = B[p.bk[k]];
+      const T xr = St::re(av), yr = St::re(bv);
+      if constexpr (CPLX) {
+        const T xi = p.conj_a ? -St::im(av) : St::im(av);
+        const T yi = p.conj_b ? -St::im(bv) : St::im(bv);
+        sr += xr * yr - xi * yi;
+        si += xr * yi + xi * yr;
+      } else {
+        sr += xr * yr;
+      }
+    }
+    T vr = alr * sr - ali * si, vi = alr * si + ali * sr;
+    S* D = (S*)p.D + od + p.dm[m] + p.dn[n];
+    const int64_t coff = p.read_c ? oc + p.cm[m] + p.cn[n] : 0;
+    const int64_t copies = p.dr == nullptr ? 1 : p.R;
+    for (int64_t r = 0; r < copies; r++) {
+      const int64_t roff = p.dr == nullptr ? 0 : p.dr[r];
+      T wr = vr, wi = vi;
+      if (p.read_c) {
+        const S cv = ((const S*)p.C)[coff + roff];
+        const T cr = St::re(cv), ci = p.conj_c ? -St::im(cv) : St::im(cv);
+        wr += ber * cr - bei * ci;
+        wi += ber * ci + bei * cr;
+      }
+      if (p.conj_d) wi = -wi;
+      D[roff] = St::make(wr, wi);
+    }
+  }
+}
+
 struct ZeroLayout {
   int rank;
   int64_t dims[8];
@@ -92,6 +167,21 @@ hipError_t launch_output_pass(const ContractArgs& a, DType t, hipStream_t s) {
     case DType::F32: hipLaunchKernelGGL((k_output_pass<float, false>), dim3(blocks), dim3(threads), 0, s, a); break;
     case DType::C64: hipLaunchKernelGGL((k_output_pass<double, true>), dim3(blocks), dim3(threads), 0, s, a); break;
     case DType::C32: hipLaunchKernelGGL((k_output_pass<float, true>), dim3(blocks), dim3(threads), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_elementwise(const ContractArgs& a, DType t, hipStream_t s) {
+  const int64_t total = a.M * a.N * a.H;
+  if (total == 0) return hipSuccess;
+  const int threads = 256;
+  int64_t blocks = (total + threads - 1) / threads;
+  if (blocks > 65536) blocks = 65536;
+  switch (t) {
+    case DType::F64: hipLaunchKernelGGL((k_elementwise<double, false>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case DType::F32: hipLaunchKernelGGL((k_elementwise<float, false>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case DType::C64: hipLaunchKernelGGL((k_elementwise<double, true>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case DType::C32: hipLaunchKernelGGL((k_elementwise<float, true>), dim3(blocks), dim3(threads), 0, s, a); break;
   }
   return hipGetLastError();
 }

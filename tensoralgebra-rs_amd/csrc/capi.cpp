@@ -162,7 +162,7 @@ void fill_report(const Plan* p, tprims_plan_report* out) {
   out->n = p->N;
   out->k = p->K;
   out->batch = p->H;
-  out->flops = p->strategy == 0 ? p->flops() : 0;
+  out->flops = p->strategy == 0 || p->strategy == 3 ? p->flops() : 0;
   out->dtype = (int32_t)p->prob.dtype;
   out->complex_method = p->cfg.complex_method;
   out->precision = p->cfg.precision;

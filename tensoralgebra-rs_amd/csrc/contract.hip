@@ -430,14 +430,19 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
   }
 }
 
+// One tile per workgroup; a grid capped below the launch limit strides over
+// the remaining tiles.
 template <class C>
 __global__ __launch_bounds__(C::NT) void k_contract(ContractArgs p) {
   __shared__ __attribute__((aligned(16))) char smem[C::LDS];
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
-  int64_t h;
-  int tm, tn;
-  dev::decode_tile(blockIdx.x, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
-  contract_tile<C>(p, h, tm, tn, smem);
+  for (int64_t t = blockIdx.x; t < nwg; t += gridDim.x) {
+    if (t != (int64_t)blockIdx.x) __syncthreads();  // the previous tile's write-back has read LDS
+    int64_t h;
+    int tm, tn;
+    dev::decode_tile(t, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
+    contract_tile<C>(p, h, tm, tn, smem);
+  }
 }
 
 template <typename T, int METH, int BM, int BN, int BK, int WM, int WN, bool BUF>
@@ -506,8 +511,8 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   p.tiles_n = (int)((p.N + C::BN - 1) / C::BN);
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
   if (nwg == 0) return hipSuccess;
-  if (nwg > 0x7fffffff) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(k_contract<C>, dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
+  const int64_t grid = nwg < max_grid(C::NT) ? nwg : max_grid(C::NT);
+  hipLaunchKernelGGL(k_contract<C>, dim3((unsigned)grid), dim3(C::NT), 0, s, p);
   return hipGetLastError();
 }
 
@@ -587,7 +592,12 @@ const TileCfg* tile_cfg(int id) {
 }
 
 hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s) {
-  if (dma_supported(a.off32, t, method, tile_id)) return launch_contract_dma(a, t, tile_id, s);
+  if (dma_supported(a.off32, t, method, tile_id)) {
+    // The DMA kernel runs one tile per workgroup, within the launch limit.
+    const TileCfg* c = tile_cfg(tile_id);
+    const int64_t nwg = ((a.M + c->bm - 1) / c->bm) * ((a.N + c->bn - 1) / c->bn) * a.H;
+    if (nwg <= max_grid(c->threads)) return launch_contract_dma(a, t, tile_id, s);
+  }
   switch (t) {
     case DType::F64: return dispatch_tile<double, M_REAL>(a, tile_id, s);
     case DType::F32: return dispatch_tile<float, M_REAL>(a, tile_id, s);

@@ -283,7 +283,7 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   p.tiles_n = (int)((p.N + C::BN - 1) / C::BN);
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
   if (nwg == 0) return hipSuccess;
-  if (nwg > 0x7fffffff) return hipErrorInvalidConfiguration;
+  if (nwg > max_grid(C::NT)) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL(k_contract_dma<C>, dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
   return hipGetLastError();
 }

@@ -196,7 +196,7 @@ void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in
   double beta[2] = {beta_in[0], beta_in[1]};
   if (!it.c) beta[0] = beta[1] = 0.0;
   const bool read_c = !is_zero(beta);
-  const bool product = p.strategy == 0 && !is_zero(alpha);
+  const bool product = (p.strategy == 0 || p.strategy == 3) && !is_zero(alpha);
   DeviceGuard dg(ex.device);
   hipStream_t s = ex.get_stream();
   const int64_t es = elem_size(p.prob.dtype);
@@ -253,6 +253,8 @@ void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in
   ContractArgs a = p.args(ex.device, wa.dev, wb.dev, wc.dev, const_cast<void*>(wd.dev), alpha, beta, read_c);
   if (!product) {
     hip_check(launch_output_pass(a, p.prob.dtype, s), "output-pass launch");
+  } else if (p.strategy == 3) {
+    hip_check(launch_elementwise(a, p.prob.dtype, s), "elementwise launch");
   } else if (p.cfg.precision == PREC_FP8) {
     std::lock_guard<std::mutex> g(ex.panel_mu);
     char* panels = ex.ensure_panels(fp8_panel_bytes(a, p.prob.dtype));

@@ -325,7 +325,7 @@ hipError_t run(const ContractArgs& a, void* panels, hipStream_t s) {
   F8PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, Bp, Bs};
   for (const F8PackArgs* q : {&pa, &pb}) {
     const int64_t grid = q->H * q->KB * (q->RB / 4);
-    if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
+    if (grid > max_grid(256)) return hipErrorInvalidConfiguration;
     if (q->vec) hipLaunchKernelGGL((k_pack_fp8<CPLX, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     else hipLaunchKernelGGL((k_pack_fp8<CPLX, false>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     hipError_t e = hipGetLastError();
@@ -335,7 +335,7 @@ hipError_t run(const ContractArgs& a, void* panels, hipStream_t s) {
   g.c.tiles_m = (int)((a.M + sh.bm - 1) / sh.bm);
   g.c.tiles_n = (int)((a.N + sh.bn - 1) / sh.bn);
   const int64_t nwg = (int64_t)g.c.tiles_m * g.c.tiles_n * a.H;
-  if (nwg > 0x7fffffff) return hipErrorInvalidConfiguration;
+  if (nwg > max_grid(512)) return hipErrorInvalidConfiguration;
   if constexpr (CPLX) {
     typedef F8Cfg<true, 128, 64, 32, 32> G;
     hipLaunchKernelGGL(k_gemm_fp8<G>, dim3((unsigned)nwg), dim3(G::NT), 0, s, g);

@@ -51,6 +51,11 @@ struct ContractArgs {
   // multiple of 64 holding the out-of-range sentinel 0x40000000.
   const uint32_t* ak32;
   const uint32_t* bk32;
+  // The batch role is one folded axis: batch offsets are h * h_stride[A, B,
+  // C, D] (used by the elementwise kernel instead of the scatter vectors).
+  int64_t h_stride[4];
+  int32_t h_affine;
+  int32_t pad1;
 };
 
 // One item of a grouped launch: the contraction plus where its tiles start in

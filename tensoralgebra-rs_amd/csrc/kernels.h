@@ -31,6 +31,10 @@ enum SplitTileId : int {
 
 const TileCfg* tile_cfg(int id);
 
+// Largest workgroup count of a one-dimensional launch of `threads`-wide
+// workgroups (work-items below 2^32), rounded down to a multiple of 8 XCDs.
+inline int64_t max_grid(int threads) { return (int64_t)((0xffffffffull / (unsigned)threads) & ~7ull); }
+
 // The contraction kernel of a storage dtype, complex method and tile.
 // Returns hipSuccess or the launch error.
 hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s);
@@ -49,6 +53,9 @@ hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int
 // 256x128 (f32), 128x64 (c32).
 size_t fp8_panel_bytes(const ContractArgs& a, DType t);
 hipError_t launch_contract_fp8(const ContractArgs& a, DType t, void* panels, hipStream_t s);
+// The elementwise (all-batch) strategy: one thread per output element, K
+// summed serially; for tiny M x N and K with many batch entries.
+hipError_t launch_elementwise(const ContractArgs& a, DType t, hipStream_t s);
 // D = op_D(beta * op_C(C)) over the output (empty K or alpha == 0).
 hipError_t launch_output_pass(const ContractArgs& a, DType t, hipStream_t s);
 // Zero every element a layout addresses (rank <= 8; used by the adjoints).

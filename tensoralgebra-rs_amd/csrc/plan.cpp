@@ -285,6 +285,7 @@ Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
   p->R = product(p->r_ax);
   if (pr.out_empty) p->strategy = 2;
   else if (pr.k_empty) p->strategy = 1;
+  else if (p->M * p->N <= 16 && p->K <= 256 && p->H * p->M * p->N >= 4096) p->strategy = 3;
   else p->strategy = 0;
   if (p->strategy == 2) {
     // Nothing is ever launched; keep the vectors trivial.
@@ -381,12 +382,20 @@ ContractArgs Plan::args(int device, const void* A, const void* B, const void* C,
   a.off32 = off32;
   a.ak32 = s.ak32;
   a.bk32 = s.bk32;
+  a.h_affine = h_ax.size() == 1;
+  if (a.h_affine) {
+    a.h_stride[0] = h_ax[0].sa;
+    a.h_stride[1] = h_ax[0].sb;
+    a.h_stride[2] = h_ax[0].sc;
+    a.h_stride[3] = h_ax[0].sd;
+  }
   return a;
 }
 
 std::string Plan::kernel_name() const {
   if (strategy == 2) return "none";
   if (strategy == 1) return std::string("output_pass.") + dtype_name(prob.dtype);
+  if (strategy == 3) return std::string("elementwise.") + dtype_name(prob.dtype);
   std::string s = dtype_name(prob.dtype);
   if (cfg.precision == PREC_SPLIT_BF16_2) s += ".split_bf16x2";
   else if (cfg.precision == PREC_SPLIT_BF16_3) s += ".split_bf16x3";

@@ -50,7 +50,9 @@ struct Plan {
   // out-of-range sentinel to a multiple of 64 entries (ContractArgs::ak32).
   std::vector<uint32_t> ak32, bk32;
   int tile = 0;
-  int strategy = 0;  // 0 contraction, 1 output pass only (K empty), 2 output empty
+  // 0 contraction (MFMA tiles), 1 output pass only (K empty), 2 output empty,
+  // 3 elementwise (all-batch: tiny M x N and K, many batch entries)
+  int strategy = 0;
 
   // Device copies, uploaded on first use per device.
   std::mutex mu;

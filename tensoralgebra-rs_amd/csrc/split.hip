@@ -369,7 +369,7 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
   PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, Bp};
   for (const PackArgs* q : {&pa, &pb}) {
     const int64_t grid = q->H * q->KB * (q->RB / 4);
-    if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
+    if (grid > max_grid(256)) return hipErrorInvalidConfiguration;
     if (q->vec) hipLaunchKernelGGL((k_pack<CPLX, TERMS, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     else hipLaunchKernelGGL((k_pack<CPLX, TERMS, false>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     hipError_t e = hipGetLastError();
@@ -379,7 +379,7 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
   g.c.tiles_m = (int)tiles_m;
   g.c.tiles_n = (int)tiles_n;
   const int64_t nwg = tiles_m * tiles_n * a.H;
-  if (nwg > 0x7fffffff) return hipErrorInvalidConfiguration;
+  if (nwg > max_grid(512)) return hipErrorInvalidConfiguration;
   if constexpr (!CPLX) {
     if (sh.bm == 256) launch_gemm<GCfg<CPLX, TERMS, 256, 128, 64, 64, 2>>(g, nwg, s);
     else if (sh.bm == 128) launch_gemm<GCfg<CPLX, TERMS, 128, 128, 64, 64, 2>>(g, nwg, s);

@@ -294,3 +294,21 @@ def test_dot_general_mirror():
     p = tp.Plan(tp.Problem.from_dot_general(tp.DType.F64, A, B, D, dg))
     r = p.report()
     assert (r["m"], r["n"], r["k"], r["batch"]) == (5, 2, 4, 3)
+
+
+def test_planner_picks_elementwise_for_all_batch_products():
+    """Hadamard-type products (tiny M x N and K, many batch entries) take the
+    elementwise strategy (crates/tprims-contract/src/strategy/elementwise.rs:194);
+    GEMM-shaped ones the MFMA tile kernels."""
+    import tprims_amd as tp
+    from harness import T, make_plan
+    n = 1 << 16
+    had = make_plan(tp.DType.F64, T([n], [1], [0]), T([n], [1], [0]), T([n], [1], [0])).report()
+    assert had["strategy"] == 3 and had["kernel"] == "elementwise.f64" and had["flops"] == 2 * n
+    # a batched 2x2 product with K = 3 is still elementwise; a 64x64 GEMM is not
+    bat = make_plan(tp.DType.C32, T([2, 3, n], [1, 2, 6], [0, 2, 3]), T([3, 2, n], [1, 3, 6], [2, 1, 3]),
+                    T([2, 2, n], [1, 2, 4], [0, 1, 3])).report()
+    assert bat["strategy"] == 3
+    gemm = make_plan(tp.DType.F32, T([64, 64], [1, 64], [0, 2]), T([64, 64], [1, 64], [2, 1]),
+                     T([64, 64], [1, 64], [0, 1])).report()
+    assert gemm["strategy"] == 0

@@ -89,3 +89,32 @@ def test_every_tile(gpu, tile, dtype):
     got, want, plan = run(dtype, config=tp.PlanConfig(tile=tile), **kw)
     assert plan.report()["tile"] == tile
     check(dtype, got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
+@pytest.mark.parametrize("layout", ["affine", "two_batch_axes", "hadamard"])
+def test_elementwise_strategy(gpu, dtype, layout):
+    """The all-batch strategy (tiny M x N and K, many batch entries) against the
+    oracle: conj(A), beta * op_C(C), one folded batch axis (affine offsets) or
+    two unfoldable ones (scatter vectors), and a pure Hadamard product."""
+    from harness import values
+    if layout == "hadamard":
+        n = 5000
+        a, b, d = T([n], [1], [0], conj=dtype.is_complex), T([n], [2], [0]), T([n], [1], [0])
+    elif layout == "affine":
+        H = 3000
+        a = T([2, 3, H], [1, 2, 6], [0, 2, 3], conj=dtype.is_complex)
+        b = T([3, 2, H], [1, 3, 6], [2, 1, 3])
+        d = T([2, 2, H], [1, 2, 4], [0, 1, 3])
+    else:
+        H1, H2 = 60, 50
+        a = T([2, 3, H1, H2], [1, 2, 6 * H2, 6], [0, 2, 3, 4], conj=dtype.is_complex)
+        b = T([3, 2, H1, H2], [1, 3, 6, 6 * H1], [2, 1, 3, 4])
+        d = T([2, 2, H1, H2], [1, 2, 4, 4 * H1], [0, 1, 3, 4])
+    av, bv = values(dtype, a, 31), values(dtype, b, 32)
+    dv = np.zeros(d.storage(), NP[dtype])
+    cv = values(dtype, d, 33)
+    got, want, plan = run(dtype, a, av, b, bv, d, dv, alpha=1.25, beta=-0.5, c=d, cv=cv)
+    assert plan.report()["strategy"] == 3
+    check(dtype, got, want)
