@@ -40,7 +40,10 @@ def parse():
     ap.add_argument("--dtype", default="c64", choices=["c64", "c32"])
     ap.add_argument("--method", default="native", choices=["native", "1m", "3m"])
     ap.add_argument("--precision", default="storage", choices=["storage", "split2", "split3", "fp8"])
-    ap.add_argument("--only", default=None, help="run only this part: main, variants, configs, cpu")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run one 1 GiB device-to-device copy (the known-size kernel that the HBM traffic "
+                         "counters are scaled by) and exit")
+    ap.add_argument("--no-vendor", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -271,6 +274,30 @@ def config_vjp(torch, tp, np, ex, stream, steps):
     return out
 
 
+def vendor_blas(torch):
+    """Same-box vendor GEMM rates (torch.matmul: hipBLASLt / rocBLAS) beside the
+    spec peaks: what the library reaches on this GPU at the same clocks."""
+    out = {}
+    for name, dt, n in (("zgemm_4096", torch.complex128, 4096), ("dgemm_4096", torch.float64, 4096),
+                        ("sgemm_4096", torch.float32, 4096), ("bf16_gemm_8192", torch.bfloat16, 8192)):
+        a = torch.randn(n, n, dtype=dt, device="cuda")
+        b = torch.randn(n, n, dtype=dt, device="cuda")
+        ms = timed(torch, lambda: a @ b, torch.cuda.current_stream(), 5, 2)
+        out[name + "_tflops"] = round((8 if dt.is_complex else 2) * n ** 3 / ms / 1e9, 1)
+        del a, b
+        torch.cuda.empty_cache()
+    return out
+
+
+def copy_calibration(torch):
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": src.numel() * 8}))
+
+
 def cpu_baseline(args):
     """The C++ port of the reference's packed driver on the host cores, timed
     on one batch entry (n = 1 of 16) of the headline contraction."""
@@ -291,6 +318,9 @@ def main():
 
     import tprims_amd as tp
 
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     stream = torch.cuda.Stream()
     ex = tp.Executor(0, stream.cuda_stream, asynchronous=True)
     w = Complex(torch, tp, args.dtype, args.method, args.precision)
@@ -372,6 +402,8 @@ def main():
             torch.cuda.empty_cache()
             print("bench: %s done" % name, file=sys.stderr, flush=True)
         out["configs"] = cfgs
+    if not args.no_vendor:
+        out["same_box_vendor_blas"] = vendor_blas(torch)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args)
     print(json.dumps(out))
 

@@ -581,6 +581,7 @@ const TileCfg* tile_cfg(int id) {
 #define TP_ROW(id, bm, bn, bk, wm, wn) {id, bm, bn, bk, (bm / wm) * (bn / wn) * 64, #id},
       TP_TILE_LIST(TP_ROW)
 #undef TP_ROW
+      {TILE_128x128, 128, 128, 16, 512, "TILE_128x128"},
       {STILE_128x128, 128, 128, 32, 256, "STILE_128x128"},
       {STILE_64x64, 64, 64, 32, 256, "STILE_64x64"},
       {STILE_256x128, 256, 128, 32, 512, "STILE_256x128"},
@@ -598,6 +599,8 @@ hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_
     const int64_t nwg = ((a.M + c->bm - 1) / c->bm) * ((a.N + c->bn - 1) / c->bn) * a.H;
     if (nwg <= max_grid(c->threads)) return launch_contract_dma(a, t, tile_id, s);
   }
+  // The DMA-only tile falls back to the register-staged 64x64 kernel.
+  if (tile_id == TILE_128x128) tile_id = TILE_64x64;
   switch (t) {
     case DType::F64: return dispatch_tile<double, M_REAL>(a, tile_id, s);
     case DType::F32: return dispatch_tile<float, M_REAL>(a, tile_id, s);

@@ -1,10 +1,10 @@
 // Storage-precision contraction kernels whose gathers are LDS DMA: every
 // lane's buffer load writes its element straight into LDS (buffer_load ... lds),
 // so the slabs never pass through registers and no VALU or LDS store
-// instruction is spent on them. Selected for f32 when every gather offset
-// fits the 32-bit buffer range (ContractArgs::off32); the template also takes
-// c64 with the native schedule, which is not selected (dma_supported). Other
-// cases run the register-staged kernels of contract.hip, with the same
+// instruction is spent on them. Selected for f32 and for c64 with the native
+// schedule when every gather offset fits the 32-bit buffer range
+// (ContractArgs::off32). Other cases run the register-staged kernels of
+// contract.hip, with the same
 // semantics (crates/tprims-kernel/src/pack/pack.rs:57-265 packing,
 // kernels/reference/scalar.rs:31,64 microkernels, pack/writeback.rs:114-495
 // write-back, driver/tile.rs:142 K accumulation).
@@ -72,7 +72,11 @@ struct DCfg {
   static constexpr int ACCS = CPLX ? 2 : 1;
   static constexpr int LDD = BN + 1;
   static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * ES;
-  static constexpr size_t LDS_EPI = (size_t)(CPLX ? 2 : 1) * BM * LDD * sizeof(T);
+  // The finished tile is staged whole when it fits in the main loop's LDS,
+  // otherwise one wave row (WM rows) at a time.
+  static constexpr size_t EPI_ROW_BYTES = (size_t)(CPLX ? 2 : 1) * LDD * sizeof(T);
+  static constexpr int EPI_ROWS = (size_t)BM * EPI_ROW_BYTES <= LDS_MAIN ? BM : WM;
+  static constexpr size_t LDS_EPI = (size_t)EPI_ROWS * EPI_ROW_BYTES;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
   static_assert(ES == 4 || ES == 16, "LDS DMA moves 4 or 16 bytes per lane");
   static_assert(64 % BK == 0 && BM % RPI == 0 && BN % RPI == 0, "K-major blocks");
@@ -259,21 +263,28 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
     if (more) issue((int)(kt & 1), a_ko, b_ko);
   }
 
-  // Stage the tile (Dre/Dim [BM][LDD]) and write it back.
+  // Stage the tile chunk by chunk (Dre/Dim [EPI_ROWS][LDD]) and write it back.
+  constexpr int ER = C::EPI_ROWS;
   T* Dre = (T*)smem;
-  T* Dim = Dre + BM * C::LDD;
+  T* Dim = Dre + ER * C::LDD;
+#pragma unroll 1
+  for (int c = 0; c < BM / ER; c++) {
+    if (c > 0) __syncthreads();  // the previous chunk's write-back has read the staging rows
+    if (wm / ER == c) {
 #pragma unroll
-  for (int i = 0; i < C::MI; i++)
+      for (int i = 0; i < C::MI; i++)
 #pragma unroll
-    for (int j = 0; j < C::NI; j++)
+        for (int j = 0; j < C::NI; j++)
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int row = wm + 16 * i + MM::row(lk, q), col = wn + 16 * j + lr;
-        Dre[row * C::LDD + col] = acc[0][i][j][q];
-        if constexpr (C::CPLX) Dim[row * C::LDD + col] = acc[1][i][j][q];
-      }
-  __syncthreads();
-  dev::write_back<T, T, C::CPLX, BM, BN, C::NT, C::LDD>(p, h, m0, n0, Dre, Dim);
+          for (int q = 0; q < 4; q++) {
+            const int row = wm - c * ER + 16 * i + MM::row(lk, q), col = wn + 16 * j + lr;
+            Dre[row * C::LDD + col] = acc[0][i][j][q];
+            if constexpr (C::CPLX) Dim[row * C::LDD + col] = acc[1][i][j][q];
+          }
+    }
+    __syncthreads();
+    dev::write_back<T, T, C::CPLX, ER, BN, C::NT, C::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
+  }
 }
 
 template <class C>
@@ -300,16 +311,22 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
 
 }  // namespace
 
-// f32 only: for c64 (native, 64x64) the DMA variant measured slower than the
-// register-staged kernel on the complex benchmark (2.44-2.50 ms against 2.35
-// ms), so the planner keeps c64 on the latter.
+// f32 and c64 with the native schedule. On the complex benchmark the c64
+// DMA kernel ran 2.41 ms against 2.98 ms for the register-staged one on a GPU
+// that holds its clock down under load (it moves no slab data through VGPRs,
+// and so draws less power), and 2.44-2.50 ms against 2.35 ms on one that does
+// not.
 bool dma_supported(bool off32, DType t, int method, int tile_id) {
-  (void)method;
   if (!off32) return false;
+  if (t == DType::C64) return method == CPLX_NATIVE && (tile_id == TILE_64x64 || tile_id == TILE_128x128);
   return t == DType::F32 && (tile_id == TILE_128x64 || tile_id == TILE_64x64);
 }
 
 hipError_t launch_contract_dma(const ContractArgs& a, DType t, int tile_id, hipStream_t s) {
+  if (t == DType::C64) {
+    if (tile_id == TILE_128x128) return launch_majors<double, true, 128, 128, 16, 64, 32>(a, s);
+    return launch_majors<double, true, 64, 64, 16, 32, 32>(a, s);
+  }
   if (t != DType::F32) return hipErrorInvalidValue;
   if (tile_id == TILE_128x64) return launch_majors<float, false, 128, 64, 16, 64, 32>(a, s);
   return launch_majors<float, false, 64, 64, 16, 32, 32>(a, s);

@@ -112,20 +112,23 @@ Axis scaled(const Axis& a, int64_t extent, int64_t by) {
   return {extent, a.sa * by, a.sb * by, a.sc * by, a.sd * by};
 }
 
-bool tile_allowed(int id, DType t, int prec) {
+bool tile_allowed(int id, DType t, int prec, int method) {
   if (prec == PREC_FP8) return id == (is_complex(t) ? STILE_128x64 : STILE_256x128);
   if (prec != PREC_STORAGE)
     return id == STILE_64x64 || (id == STILE_128x64 && is_complex(t)) ||
            ((id == STILE_128x128 || id == STILE_256x128) && !is_complex(t));
   if (id >= 10) return false;
-  if (id == TILE_128x64 && t == DType::C64) return false;  // staging would exceed 64 KiB
+  // c64 keeps the 64x64 register-staged tile and gets the 128x128 DMA tile
+  // instead of 128x64; 128x128 exists for c64 only.
+  if (id == TILE_128x64 && t == DType::C64) return false;
+  if (id == TILE_128x128) return t == DType::C64 && method == CPLX_NATIVE;
   return tile_cfg(id) != nullptr;
 }
 
 int choose_tile(const Plan& p) {
   const int prec = p.cfg.precision;
   if (p.cfg.tile != 0) {
-    if (!tile_allowed(p.cfg.tile, p.prob.dtype, prec)) fail(ERR_UNSUPPORTED, "tile id not available for this plan");
+    if (!tile_allowed(p.cfg.tile, p.prob.dtype, prec, p.cfg.complex_method)) fail(ERR_UNSUPPORTED, "tile id not available for this plan");
     return p.cfg.tile;
   }
   std::vector<std::pair<int, double>> cands;
@@ -134,12 +137,12 @@ int choose_tile(const Plan& p) {
   } else if (prec != PREC_STORAGE) {
     cands = {{STILE_256x128, 1.3}, {STILE_128x128, 1.15}, {STILE_128x64, 1.2}, {STILE_64x64, 1.0}};
   } else {
-    cands = {{TILE_128x64, 1.1}, {TILE_64x64, 1.0}, {TILE_32x32, 0.7}};
+    cands = {{TILE_128x128, 1.3}, {TILE_128x64, 1.1}, {TILE_64x64, 1.0}, {TILE_32x32, 0.7}};
   }
   int best = 0;
   double best_score = -1;
   for (auto [id, eff] : cands) {
-    if (!tile_allowed(id, p.prob.dtype, prec)) continue;
+    if (!tile_allowed(id, p.prob.dtype, prec, p.cfg.complex_method)) continue;
     const TileCfg* t = tile_cfg(id);
     double wgs = (double)((p.M + t->bm - 1) / t->bm) * (double)((p.N + t->bn - 1) / t->bn) * (double)p.H;
     double waves = wgs * (t->threads / 64);
@@ -256,6 +259,9 @@ void configure(Plan* plan, const PlanConfig& cfg) {
     return *rlo >= 0 && *klo >= 0 && (__int128)(*rhi + *khi + 1) * es <= (__int128)(1 << 30);
   };
   p.off32 = fits32(p.am, p.ak) && fits32(p.bn, p.bk);
+  // The 128x128 tile runs on the DMA kernel only, which needs 32-bit offsets
+  // (the K order is the same for both tiles: BK 16).
+  if (p.tile == TILE_128x128 && !p.off32 && cfg.tile == 0) p.tile = TILE_64x64;
   p.ak32.clear();
   p.bk32.clear();
   if (p.off32) {
@@ -405,9 +411,11 @@ std::string Plan::kernel_name() const {
     static const char* m[3] = {".native", ".1m", ".3m"};
     s += m[cfg.complex_method];
   }
-  const TileCfg* t = tile_cfg(tile);
+  // The DMA-only tile runs as 64x64 on the register-staged kernel without 32-bit offsets.
+  const bool fallback = tile == TILE_128x128 && !dma_supported(off32, prob.dtype, cfg.complex_method, tile);
+  const TileCfg* t = tile_cfg(fallback ? TILE_64x64 : tile);
   if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(cfg.precision == PREC_FP8 ? 128 : t->bk);
-  if (cfg.precision == PREC_STORAGE && dma_supported(off32, prob.dtype, cfg.complex_method, tile)) s += ".dma";
+  if (cfg.precision == PREC_STORAGE && !fallback && dma_supported(off32, prob.dtype, cfg.complex_method, tile)) s += ".dma";
   return s;
 }
 

@@ -80,11 +80,13 @@ def test_host_operands_are_staged(gpu, name, dtype):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tile", [1, 2, 3])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4])
 @pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
 def test_every_tile(gpu, tile, dtype):
     if tile == 3 and dtype == tp.DType.C64:
         pytest.skip("tile 3 is not offered for c64")
+    if tile == 4 and dtype != tp.DType.C64:
+        pytest.skip("tile 4 (128x128, LDS-DMA) is offered for c64 only")
     kw = CASES["large_k"](dtype)
     got, want, plan = run(dtype, config=tp.PlanConfig(tile=tile), **kw)
     assert plan.report()["tile"] == tile
@@ -118,3 +120,16 @@ def test_elementwise_strategy(gpu, dtype, layout):
     got, want, plan = run(dtype, a, av, b, bv, d, dv, alpha=1.25, beta=-0.5, c=d, cv=cv)
     assert plan.report()["strategy"] == 3
     check(dtype, got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_c64_128x128_dma_tile_every_case(gpu, name):
+    """The c64 128x128 tile (8 waves of 64x32 on the LDS-DMA kernel; the
+    register-staged 64x64 kernel when an operand needs 64-bit offsets) on
+    every conformance case."""
+    kw = CASES[name](tp.DType.C64)
+    got, want, plan = run(tp.DType.C64, config=tp.PlanConfig(tile=4), **kw)
+    if plan.report()["strategy"] == 0:
+        assert plan.report()["tile"] == 4
+    check(tp.DType.C64, got, want)
