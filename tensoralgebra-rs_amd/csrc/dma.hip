@@ -27,6 +27,8 @@
 // stage kt & 1 retired), then the DMA of slab kt + 2 into stage kt & 1.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -58,33 +60,41 @@ struct DCfg {
   static constexpr int NW = (BM / WM) * (BN / WN);
   static constexpr int NT = NW * 64;
   typedef typename Store<T, CPLX>::S S;
-  static constexpr int ES = sizeof(S);  // 4 (f32) or 16 (c64): the DMA sizes used
+  static constexpr int ES = sizeof(S);  // element bytes in memory
+  // c32 elements (8 bytes, no DMA size) arrive as two 4-byte DMAs into
+  // separate re and im planes; f32 and c64 elements as one DMA each.
+  static constexpr bool PLANAR = CPLX && sizeof(T) == 4;
+  static constexpr int PLANES = PLANAR ? 2 : 1;
+  typedef typename std::conditional<PLANAR, float, S>::type W;  // LDS element
+  static constexpr int EW = sizeof(W);  // DMA size: 4 or 16
   static constexpr int RPI = 64 / BK;   // rows per K-major block
-  static constexpr int PADB = ES == 16 ? 1 : 4;
-  static constexpr int SW = ES == 16 ? 4 : 0;
-  static constexpr int PADM = ES == 16 ? 0 : 16;
+  static constexpr int PADB = EW == 16 ? 1 : 4;
+  static constexpr int SW = EW == 16 ? 4 : 0;
+  static constexpr int PADM = EW == 16 ? 0 : 16;
   static constexpr int A_ELEMS = AKM ? (BM / RPI) * (64 + PADB) : BK * (BM + PADM);
   static constexpr int B_ELEMS = BKM ? (BN / RPI) * (64 + PADB) : BK * (BN + PADM);
-  static constexpr int STAGE = A_ELEMS + B_ELEMS;  // elements of one stage
+  static constexpr int PLANE = A_ELEMS + B_ELEMS;  // LDS elements of one plane
+  static constexpr int STAGE = PLANES * PLANE;     // LDS elements of one stage
   static constexpr int A_INSTR = BM * BK / 64, B_INSTR = BN * BK / 64;
   static constexpr int JA = A_INSTR / NW, JB = B_INSTR / NW;  // DMA instructions per wave per slab
   static constexpr int MI = WM / 16, NI = WN / 16;
-  static constexpr int ACCS = CPLX ? 2 : 1;
+  // Accumulators: real 1; c64 2 (re, im; the f64 MFMA negates -im*im for
+  // free); c32 3 (re*re, im*im subtracted at the end, cross).
+  static constexpr int ACCS = !CPLX ? 1 : (PLANAR ? 3 : 2);
   static constexpr int LDD = BN + 1;
-  static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * ES;
+  static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * EW;
   // The finished tile is staged whole when it fits in the main loop's LDS,
   // otherwise one wave row (WM rows) at a time.
   static constexpr size_t EPI_ROW_BYTES = (size_t)(CPLX ? 2 : 1) * LDD * sizeof(T);
   static constexpr int EPI_ROWS = (size_t)BM * EPI_ROW_BYTES <= LDS_MAIN ? BM : WM;
   static constexpr size_t LDS_EPI = (size_t)EPI_ROWS * EPI_ROW_BYTES;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
-  static_assert(ES == 4 || ES == 16, "LDS DMA moves 4 or 16 bytes per lane");
+  static_assert(EW == 4 || EW == 16, "LDS DMA moves 4 or 16 bytes per lane");
   static_assert(64 % BK == 0 && BM % RPI == 0 && BN % RPI == 0, "K-major blocks");
   static_assert(BM % 64 == 0 || AKM, "M-major rows are whole DMA instructions");
   static_assert(BN % 64 == 0 || BKM, "N-major rows are whole DMA instructions");
   static_assert(A_INSTR % NW == 0 && B_INSTR % NW == 0, "whole DMA instructions per wave");
   static_assert(BK % 4 == 0 && WM % 16 == 0 && WN % 16 == 0, "MFMA shape");
-  static_assert(!CPLX || sizeof(T) == 8, "complex DMA path is c64 (MFMA negate flag)");
   static_assert(LDS <= 160 * 1024, "LDS budget");
 
   // LDS slot of element (row r, k) of a stage's A (rows BM) or B (rows BN) part.
@@ -98,12 +108,12 @@ struct DCfg {
 template <class C>
 __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
   typedef typename C::Tv T;
-  typedef typename C::S S;
+  typedef typename C::W W;
   typedef Mma<T> MM;
   typedef typename MM::acc Acc;
   constexpr int BM = C::BM, BN = C::BN, BK = C::BK, NW = C::NW;
   __shared__ __attribute__((aligned(16))) char smem[C::LDS];
-  S* const lds = (S*)smem;
+  W* const lds = (W*)smem;
 
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
   int64_t h;
@@ -115,6 +125,7 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
   const int wn = (wave / C::WAVES_M) * C::WN;
   const int64_t M = p.M, N = p.N, K = p.K;
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  typedef typename C::S S;
   const __amdgpu_buffer_rsrc_t rsA = dev::make_rsrc((const S*)p.A + p.ha[h]);
   const __amdgpu_buffer_rsrc_t rsB = dev::make_rsrc((const S*)p.B + p.hb[h]);
 
@@ -170,12 +181,16 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
 #pragma unroll
     for (int t = 0; t < C::JB; t++) kb[t] = p.bk32[k0 + b_kl[t]];
   };
+  // Plane c of an element holds its bytes [4 c, 4 c + EW) (re, then im for c32).
   auto issue = [&](int st, const uint32_t* ka, const uint32_t* kb) {
-    S* stage = lds + st * C::STAGE;
 #pragma unroll
-    for (int t = 0; t < C::JA; t++) dma_load<C::ES>(rsA, stage + a_dst(t), a_row[t] + ka[t]);
+    for (int c = 0; c < C::PLANES; c++) {
+      W* stage = lds + st * C::STAGE + c * C::PLANE;
 #pragma unroll
-    for (int t = 0; t < C::JB; t++) dma_load<C::ES>(rsB, stage + b_dst(t), b_row[t] + kb[t]);
+      for (int t = 0; t < C::JA; t++) dma_load<C::EW>(rsA, stage + a_dst(t), a_row[t] + ka[t] + 4 * c);
+#pragma unroll
+      for (int t = 0; t < C::JB; t++) dma_load<C::EW>(rsB, stage + b_dst(t), b_row[t] + kb[t] + 4 * c);
+    }
   };
 
   Acc acc[C::ACCS][C::MI][C::NI];
@@ -192,12 +207,12 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
     if constexpr (sizeof(T) == 8) {
       return __builtin_bit_cast(T, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)sign << 32));
     } else {
-      return x;
+      return __builtin_bit_cast(T, __builtin_bit_cast(uint32_t, x) ^ sign);
     }
   };
   auto compute = [&](int st) {
-    const S* As = lds + st * C::STAGE;
-    const S* Bs = As + C::A_ELEMS;
+    const W* As = lds + st * C::STAGE;
+    const W* Bs = As + C::A_ELEMS;
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       if constexpr (!C::CPLX) {
@@ -214,30 +229,45 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
         T ar[C::MI], ai[C::MI], br[C::NI], bi[C::NI];
 #pragma unroll
         for (int i = 0; i < C::MI; i++) {
-          const S v = As[C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk)];
-          ar[i] = v.re;
-          ai[i] = flip(v.im, ca);
+          const int x = C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk);
+          if constexpr (C::PLANAR) {
+            ar[i] = As[x];
+            ai[i] = flip(As[C::PLANE + x], ca);
+          } else {
+            const S v = As[x];
+            ar[i] = v.re;
+            ai[i] = flip(v.im, ca);
+          }
         }
 #pragma unroll
         for (int j = 0; j < C::NI; j++) {
-          const S v = Bs[C::template slot<C::BKM, BN>(wn + 16 * j + lr, kk + lk)];
-          br[j] = v.re;
-          bi[j] = flip(v.im, cb);
+          const int x = C::template slot<C::BKM, BN>(wn + 16 * j + lr, kk + lk);
+          if constexpr (C::PLANAR) {
+            br[j] = Bs[x];
+            bi[j] = flip(Bs[C::PLANE + x], cb);
+          } else {
+            const S v = Bs[x];
+            br[j] = v.re;
+            bi[j] = flip(v.im, cb);
+          }
         }
-        // acc0 += ar br - ai bi, acc1 += ar bi + ai br
+        // c64: acc0 += ar br - ai bi, acc1 += ar bi + ai br. c32: acc0 += ar br,
+        // acc1 += ai bi (subtracted at the end), acc2 += ar bi + ai br.
+        constexpr int XI = C::PLANAR ? 2 : 1;
 #pragma unroll
         for (int i = 0; i < C::MI; i++)
 #pragma unroll
           for (int j = 0; j < C::NI; j++) {
             acc[0][i][j] = MM::run(ar[i], br[j], acc[0][i][j]);
-            acc[1][i][j] = MM::run(ar[i], bi[j], acc[1][i][j]);
+            acc[XI][i][j] = MM::run(ar[i], bi[j], acc[XI][i][j]);
           }
 #pragma unroll
         for (int i = 0; i < C::MI; i++)
 #pragma unroll
           for (int j = 0; j < C::NI; j++) {
-            acc[0][i][j] = MM::run_neg_a(ai[i], bi[j], acc[0][i][j]);
-            acc[1][i][j] = MM::run(ai[i], br[j], acc[1][i][j]);
+            if constexpr (C::PLANAR) acc[1][i][j] = MM::run(ai[i], bi[j], acc[1][i][j]);
+            else acc[0][i][j] = MM::run_neg_a(ai[i], bi[j], acc[0][i][j]);
+            acc[XI][i][j] = MM::run(ai[i], br[j], acc[XI][i][j]);
           }
       }
     }
@@ -278,8 +308,13 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
 #pragma unroll
           for (int q = 0; q < 4; q++) {
             const int row = wm - c * ER + 16 * i + MM::row(lk, q), col = wn + 16 * j + lr;
-            Dre[row * C::LDD + col] = acc[0][i][j][q];
-            if constexpr (C::CPLX) Dim[row * C::LDD + col] = acc[1][i][j][q];
+            if constexpr (C::PLANAR) {
+              Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
+              Dim[row * C::LDD + col] = acc[2][i][j][q];
+            } else {
+              Dre[row * C::LDD + col] = acc[0][i][j][q];
+              if constexpr (C::CPLX) Dim[row * C::LDD + col] = acc[1][i][j][q];
+            }
           }
     }
     __syncthreads();
@@ -319,7 +354,9 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
 bool dma_supported(bool off32, DType t, int method, int tile_id) {
   if (!off32) return false;
   if (t == DType::C64) return method == CPLX_NATIVE && (tile_id == TILE_64x64 || tile_id == TILE_128x128);
-  return t == DType::F32 && (tile_id == TILE_128x64 || tile_id == TILE_64x64);
+  if (t == DType::C32)
+    return method == CPLX_NATIVE && (tile_id == TILE_64x64 || tile_id == TILE_128x64 || tile_id == TILE_128x128);
+  return t == DType::F32 && (tile_id == TILE_128x128 || tile_id == TILE_128x64 || tile_id == TILE_64x64);
 }
 
 hipError_t launch_contract_dma(const ContractArgs& a, DType t, int tile_id, hipStream_t s) {
@@ -327,7 +364,13 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int tile_id, hipS
     if (tile_id == TILE_128x128) return launch_majors<double, true, 128, 128, 16, 64, 32>(a, s);
     return launch_majors<double, true, 64, 64, 16, 32, 32>(a, s);
   }
+  if (t == DType::C32) {
+    if (tile_id == TILE_128x128) return launch_majors<float, true, 128, 128, 16, 64, 32>(a, s);
+    if (tile_id == TILE_128x64) return launch_majors<float, true, 128, 64, 16, 64, 32>(a, s);
+    return launch_majors<float, true, 64, 64, 16, 32, 32>(a, s);
+  }
   if (t != DType::F32) return hipErrorInvalidValue;
+  if (tile_id == TILE_128x128) return launch_majors<float, false, 128, 128, 16, 64, 32>(a, s);
   if (tile_id == TILE_128x64) return launch_majors<float, false, 128, 64, 16, 64, 32>(a, s);
   return launch_majors<float, false, 64, 64, 16, 32, 32>(a, s);
 }

@@ -119,9 +119,9 @@ bool tile_allowed(int id, DType t, int prec, int method) {
            ((id == STILE_128x128 || id == STILE_256x128) && !is_complex(t));
   if (id >= 10) return false;
   // c64 keeps the 64x64 register-staged tile and gets the 128x128 DMA tile
-  // instead of 128x64; 128x128 exists for c64 only.
+  // instead of 128x64; 128x128 exists for f32 and the native complex schedule.
   if (id == TILE_128x64 && t == DType::C64) return false;
-  if (id == TILE_128x128) return t == DType::C64 && method == CPLX_NATIVE;
+  if (id == TILE_128x128) return t == DType::F32 || (is_complex(t) && method == CPLX_NATIVE);
   return tile_cfg(id) != nullptr;
 }
 

@@ -85,8 +85,8 @@ def test_host_operands_are_staged(gpu, name, dtype):
 def test_every_tile(gpu, tile, dtype):
     if tile == 3 and dtype == tp.DType.C64:
         pytest.skip("tile 3 is not offered for c64")
-    if tile == 4 and dtype != tp.DType.C64:
-        pytest.skip("tile 4 (128x128, LDS-DMA) is offered for c64 only")
+    if tile == 4 and dtype == tp.DType.F64:
+        pytest.skip("tile 4 (128x128, LDS-DMA) is offered for f32, c32 and c64 only")
     kw = CASES["large_k"](dtype)
     got, want, plan = run(dtype, config=tp.PlanConfig(tile=tile), **kw)
     assert plan.report()["tile"] == tile
@@ -123,13 +123,14 @@ def test_elementwise_strategy(gpu, dtype, layout):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.C32, tp.DType.C64], ids=lambda d: d.name)
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_c64_128x128_dma_tile_every_case(gpu, name):
-    """The c64 128x128 tile (8 waves of 64x32 on the LDS-DMA kernel; the
+def test_128x128_dma_tile_every_case(gpu, name, dtype):
+    """The 128x128 tile (8 waves of 64x32 on the LDS-DMA kernel; the
     register-staged 64x64 kernel when an operand needs 64-bit offsets) on
     every conformance case."""
-    kw = CASES[name](tp.DType.C64)
-    got, want, plan = run(tp.DType.C64, config=tp.PlanConfig(tile=4), **kw)
+    kw = CASES[name](dtype)
+    got, want, plan = run(dtype, config=tp.PlanConfig(tile=4), **kw)
     if plan.report()["strategy"] == 0:
         assert plan.report()["tile"] == 4
-    check(tp.DType.C64, got, want)
+    check(dtype, got, want)
