@@ -63,8 +63,6 @@ __global__ void k_elementwise(ContractArgs p) {
   typedef typename St::S S;
   const int64_t M = p.M, N = p.N, H = p.H, K = p.K;
   const int64_t total = M * N * H;
-  const T alr = (T)p.alpha[0], ali = (T)p.alpha[1];
-  const T ber = (T)p.beta[0], bei = (T)p.beta[1];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     int64_t m, n, h;
     if (M * N == 1) {
@@ -81,18 +79,8 @@ __global__ void k_elementwise(ContractArgs p) {
       m = r % M;
       h = r / M;
     }
-    int64_t oa, ob, oc, od;
-    if (p.h_affine) {
-      oa = h * p.h_stride[0];
-      ob = h * p.h_stride[1];
-      oc = h * p.h_stride[2];
-      od = h * p.h_stride[3];
-    } else {
-      oa = p.ha[h];
-      ob = p.hb[h];
-      oc = p.read_c ? p.hc[h] : 0;
-      od = p.hd[h];
-    }
+    const int64_t oa = p.h_affine ? h * p.h_stride[0] : p.ha[h];
+    const int64_t ob = p.h_affine ? h * p.h_stride[1] : p.hb[h];
     const S* A = (const S*)p.A + oa + p.am[m];
     const S* B = (const S*)p.B + ob + p.bn[n];
     T sr = 0, si = 0;
@@ -108,22 +96,28 @@ __global__ void k_elementwise(ContractArgs p) {
         sr += xr * yr;
       }
     }
-    T vr = alr * sr - ali * si, vi = alr * si + ali * sr;
-    S* D = (S*)p.D + od + p.dm[m] + p.dn[n];
-    const int64_t coff = p.read_c ? oc + p.cm[m] + p.cn[n] : 0;
-    const int64_t copies = p.dr == nullptr ? 1 : p.R;
-    for (int64_t r = 0; r < copies; r++) {
-      const int64_t roff = p.dr == nullptr ? 0 : p.dr[r];
-      T wr = vr, wi = vi;
-      if (p.read_c) {
-        const S cv = ((const S*)p.C)[coff + roff];
-        const T cr = St::re(cv), ci = p.conj_c ? -St::im(cv) : St::im(cv);
-        wr += ber * cr - bei * ci;
-        wi += ber * ci + bei * cr;
-      }
-      if (p.conj_d) wi = -wi;
-      D[roff] = St::make(wr, wi);
+    dev::store_element<T, CPLX>(p, h, m, n, sr, si);
+  }
+}
+
+// Split-K reduction: element (h, m, n) sums the ksplit partials of the
+// workspace in split order (deterministic) and does the write-back.
+template <typename T, bool CPLX>
+__global__ void k_splitk_reduce(ContractArgs p) {
+  typedef Store<T, CPLX> St;
+  typedef typename St::S S;
+  const int64_t per = p.M * p.N, total = per * p.H;
+  const S* W = (const S*)p.work;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n = e % p.N, r = e / p.N;
+    const int64_t m = r % p.M, h = r / p.M;
+    T sr = 0, si = 0;
+    for (int s = 0; s < p.ksplit; s++) {
+      const S v = W[(int64_t)s * total + e];
+      sr += St::re(v);
+      si += St::im(v);
     }
+    dev::store_element<T, CPLX>(p, h, m, n, sr, si);
   }
 }
 
@@ -182,6 +176,21 @@ hipError_t launch_elementwise(const ContractArgs& a, DType t, hipStream_t s) {
     case DType::F32: hipLaunchKernelGGL((k_elementwise<float, false>), dim3(blocks), dim3(threads), 0, s, a); break;
     case DType::C64: hipLaunchKernelGGL((k_elementwise<double, true>), dim3(blocks), dim3(threads), 0, s, a); break;
     case DType::C32: hipLaunchKernelGGL((k_elementwise<float, true>), dim3(blocks), dim3(threads), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_splitk_reduce(const ContractArgs& a, DType t, hipStream_t s) {
+  const int64_t total = a.M * a.N * a.H;
+  if (total == 0) return hipSuccess;
+  const int threads = 256;
+  int64_t blocks = (total + threads - 1) / threads;
+  if (blocks > 65536) blocks = 65536;
+  switch (t) {
+    case DType::F64: hipLaunchKernelGGL((k_splitk_reduce<double, false>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case DType::F32: hipLaunchKernelGGL((k_splitk_reduce<float, false>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case DType::C64: hipLaunchKernelGGL((k_splitk_reduce<double, true>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case DType::C32: hipLaunchKernelGGL((k_splitk_reduce<float, true>), dim3(blocks), dim3(threads), 0, s, a); break;
   }
   return hipGetLastError();
 }

@@ -103,7 +103,7 @@ struct Cfg {
 // One tile of one contraction, the body shared by the single-contraction
 // kernel and the grouped persistent kernel.
 template <class C>
-__device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, int tn, char* smem) {
+__device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, int tn, char* smem, int split = 0) {
   typedef typename C::Tv T;
   typedef Store<T, C::CPLX> St;
   typedef typename St::S S;
@@ -270,14 +270,19 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
       for (int j = 0; j < C::NI; j++) acc[s][i][j] = Acc{};
 
   const int lr = lane & 15, lk = lane >> 4;
-  const int64_t nk = (K + BK - 1) / BK;
-  TP_LOAD_KOFF(0);
+  // This workgroup's K slabs: all of them, or split `split`'s share.
+  const int64_t nk_all = (K + BK - 1) / BK;
+  const int64_t kt0 = p.ksplit > 1 ? (int64_t)split * p.kslabs : 0;
+  const int64_t nk_rest = nk_all - kt0;
+  const int64_t nk = p.ksplit > 1 ? (nk_rest < p.kslabs ? (nk_rest > 0 ? nk_rest : 0) : p.kslabs) : nk_all;
+  const int64_t kb0 = kt0 * BK;  // first k of the range
+  TP_LOAD_KOFF(kb0);
   TP_GATHER();
-  if (nk > 1) TP_LOAD_KOFF(BK);
+  if (nk > 1) TP_LOAD_KOFF(kb0 + BK);
   stash(0);
   if (nk > 1) {
     TP_GATHER();
-    if (nk > 2) TP_LOAD_KOFF(2 * BK);
+    if (nk > 2) TP_LOAD_KOFF(kb0 + 2 * BK);
   }
   __syncthreads();
   for (int64_t kt = 0; kt < nk; kt++) {
@@ -379,7 +384,7 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
     // lane's safe address (K offsets clamp to K - 1, gathers to element 0).
     stash((int)((kt + 1) & 1));
     TP_GATHER();
-    TP_LOAD_KOFF((kt + 3) * BK);
+    TP_LOAD_KOFF(kb0 + (kt + 3) * BK);
     __syncthreads();
   }
 #undef TP_LOAD_KOFF
@@ -426,7 +431,10 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
       }
     }
     __syncthreads();
-    dev::write_back<T, T, C::CPLX, ER, BN, NT, C::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
+    if (p.ksplit > 1)
+      dev::write_partial<T, C::CPLX, ER, BN, NT, C::LDD>(p, split, h, m0 + c * ER, n0, Dre, Dim);
+    else
+      dev::write_back<T, T, C::CPLX, ER, BN, NT, C::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
   }
 }
 
@@ -436,12 +444,14 @@ template <class C>
 __global__ __launch_bounds__(C::NT) void k_contract(ContractArgs p) {
   __shared__ __attribute__((aligned(16))) char smem[C::LDS];
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
-  for (int64_t t = blockIdx.x; t < nwg; t += gridDim.x) {
+  // With split K the splits of a tile are ksplit consecutive ids.
+  const int S = p.ksplit > 1 ? p.ksplit : 1;
+  for (int64_t t = blockIdx.x; t < nwg * S; t += gridDim.x) {
     if (t != (int64_t)blockIdx.x) __syncthreads();  // the previous tile's write-back has read LDS
     int64_t h;
     int tm, tn;
-    dev::decode_tile(t, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
-    contract_tile<C>(p, h, tm, tn, smem);
+    dev::decode_tile(t / S, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
+    contract_tile<C>(p, h, tm, tn, smem, (int)(t % S));
   }
 }
 
@@ -509,7 +519,7 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   ContractArgs p = a;
   p.tiles_m = (int)((p.M + C::BM - 1) / C::BM);
   p.tiles_n = (int)((p.N + C::BN - 1) / C::BN);
-  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
+  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H * (p.ksplit > 1 ? p.ksplit : 1);
   if (nwg == 0) return hipSuccess;
   const int64_t grid = nwg < max_grid(C::NT) ? nwg : max_grid(C::NT);
   hipLaunchKernelGGL(k_contract<C>, dim3((unsigned)grid), dim3(C::NT), 0, s, p);
@@ -581,7 +591,7 @@ const TileCfg* tile_cfg(int id) {
 #define TP_ROW(id, bm, bn, bk, wm, wn) {id, bm, bn, bk, (bm / wm) * (bn / wn) * 64, #id},
       TP_TILE_LIST(TP_ROW)
 #undef TP_ROW
-      {TILE_128x128, 128, 128, 16, 512, "TILE_128x128"},
+      {TILE_128x128, 128, 128, 16, 512, "TILE_128x128"},  // c64: 1024 threads
       {STILE_128x128, 128, 128, 32, 256, "STILE_128x128"},
       {STILE_64x64, 64, 64, 32, 256, "STILE_64x64"},
       {STILE_256x128, 256, 128, 32, 512, "STILE_256x128"},
@@ -593,7 +603,7 @@ const TileCfg* tile_cfg(int id) {
 }
 
 hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s) {
-  if (dma_supported(a.off32, t, method, tile_id)) {
+  if (a.ksplit <= 1 && dma_supported(a.off32, t, method, tile_id)) {
     // The DMA kernel runs one tile per workgroup, within the launch limit.
     const TileCfg* c = tile_cfg(tile_id);
     const int64_t nwg = ((a.M + c->bm - 1) / c->bm) * ((a.N + c->bn - 1) / c->bn) * a.H;

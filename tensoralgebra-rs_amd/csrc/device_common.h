@@ -103,6 +103,55 @@ __device__ inline void decode_tile(int64_t bid, int64_t nwg, int tiles_m, int ti
   tn = r / gsize;
 }
 
+// Writes a staged BM x BN partial tile of split `split` (Dre/Dim, row pitch
+// LDD, in LDS) unscaled to the split-K workspace work[split][h][m][n], rows
+// past M and columns past N skipped; lanes run along n.
+template <typename T, bool CPLX, int BM, int BN, int NT, int LDD>
+__device__ inline void write_partial(const ContractArgs& p, int split, int64_t h, int64_t m0, int64_t n0, const T* Dre,
+                                     const T* Dim) {
+  typedef Store<T, CPLX> St;
+  typedef typename St::S S;
+  S* W = (S*)p.work + (((int64_t)split * p.H + h) * p.M) * p.N;
+  constexpr int EPI = BM * BN / NT;
+#pragma unroll 4
+  for (int t = 0; t < EPI; t++) {
+    const int e = threadIdx.x + t * NT;
+    const int ml = e / BN, nl = e % BN;
+    const int64_t m = m0 + ml, n = n0 + nl;
+    if (m < p.M && n < p.N) W[m * p.N + n] = St::make(Dre[ml * LDD + nl], CPLX ? Dim[ml * LDD + nl] : T(0));
+  }
+}
+
+// D[h, m, n] = op_D(alpha * s + beta * op_C(C)) for one element given its sum
+// s = (sr, si), with the broadcast copies of an adjoint (C is D there and each
+// copy reads its own previous value). Offsets of h come from the scatter
+// vectors or, for one folded batch axis, from h * stride.
+template <typename T, bool CPLX>
+__device__ inline void store_element(const ContractArgs& p, int64_t h, int64_t m, int64_t n, T sr, T si) {
+  typedef Store<T, CPLX> St;
+  typedef typename St::S S;
+  const T alr = (T)p.alpha[0], ali = (T)p.alpha[1];
+  const T ber = (T)p.beta[0], bei = (T)p.beta[1];
+  const int64_t oc = p.read_c ? (p.h_affine ? h * p.h_stride[2] : p.hc[h]) : 0;
+  const int64_t od = p.h_affine ? h * p.h_stride[3] : p.hd[h];
+  const T vr = alr * sr - ali * si, vi = alr * si + ali * sr;
+  S* D = (S*)p.D + od + p.dm[m] + p.dn[n];
+  const int64_t coff = p.read_c ? oc + p.cm[m] + p.cn[n] : 0;
+  const int64_t copies = p.dr == nullptr ? 1 : p.R;
+  for (int64_t r = 0; r < copies; r++) {
+    const int64_t roff = p.dr == nullptr ? 0 : p.dr[r];
+    T wr = vr, wi = vi;
+    if (p.read_c) {
+      const S cv = ((const S*)p.C)[coff + roff];
+      const T cr = St::re(cv), ci = p.conj_c ? -St::im(cv) : St::im(cv);
+      wr += ber * cr - bei * ci;
+      wi += ber * ci + bei * cr;
+    }
+    if (p.conj_d) wi = -wi;
+    D[roff] = St::make(wr, wi);
+  }
+}
+
 // Writes a staged BM x BN tile (Dre/Dim, row pitch LDD, in LDS) to D along
 // D's contiguous axis: alpha scaling, beta * op_C(C) only when beta != 0 (C is
 // never read otherwise), op_D last, and the broadcast copies of an adjoint.

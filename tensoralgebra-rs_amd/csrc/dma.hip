@@ -346,11 +346,13 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
 
 }  // namespace
 
-// f32 and c64 with the native schedule. On the complex benchmark the c64
-// DMA kernel ran 2.41 ms against 2.98 ms for the register-staged one on a GPU
-// that holds its clock down under load (it moves no slab data through VGPRs,
-// and so draws less power), and 2.44-2.50 ms against 2.35 ms on one that does
-// not.
+// f32, c32 and c64 with the native schedule. On the complex benchmark the
+// c64 64x64 DMA kernel ran 2.41 ms against 2.98 ms for the register-staged
+// one on a GPU that holds its clock down under load (it moves no slab data
+// through VGPRs, and so draws less power), and 2.44-2.50 ms against 2.35 ms on
+// one that does not; the 128x128 tile brings it to 2.20 ms with 8 waves of
+// 64x32 and 2.13 ms with 16 waves of 32x32 (c64's choice; f32 and c32 keep 8
+// waves of 64x32).
 bool dma_supported(bool off32, DType t, int method, int tile_id) {
   if (!off32) return false;
   if (t == DType::C64) return method == CPLX_NATIVE && (tile_id == TILE_64x64 || tile_id == TILE_128x128);
@@ -361,7 +363,7 @@ bool dma_supported(bool off32, DType t, int method, int tile_id) {
 
 hipError_t launch_contract_dma(const ContractArgs& a, DType t, int tile_id, hipStream_t s) {
   if (t == DType::C64) {
-    if (tile_id == TILE_128x128) return launch_majors<double, true, 128, 128, 16, 64, 32>(a, s);
+    if (tile_id == TILE_128x128) return launch_majors<double, true, 128, 128, 16, 32, 32>(a, s);
     return launch_majors<double, true, 64, 64, 16, 32, 32>(a, s);
   }
   if (t == DType::C32) {

@@ -264,6 +264,13 @@ void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in
     std::lock_guard<std::mutex> g(ex.panel_mu);
     char* panels = ex.ensure_panels(split_panel_bytes(a, p.prob.dtype, terms, p.tile));
     hip_check(launch_contract_split(a, p.prob.dtype, terms, p.tile, panels, s), "split-BF16 contraction launch");
+  } else if (p.ksplit > 1) {
+    std::lock_guard<std::mutex> g(ex.panel_mu);
+    a.ksplit = p.ksplit;
+    a.kslabs = p.kslabs;
+    a.work = ex.ensure_panels((size_t)p.ksplit * (size_t)(p.M * p.N * p.H) * (size_t)es);
+    hip_check(launch_contract(a, p.prob.dtype, p.cfg.complex_method, p.tile, s), "contraction launch");
+    hip_check(launch_splitk_reduce(a, p.prob.dtype, s), "split-K reduction launch");
   } else {
     hip_check(launch_contract(a, p.prob.dtype, p.cfg.complex_method, p.tile, s), "contraction launch");
   }

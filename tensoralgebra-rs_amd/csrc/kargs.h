@@ -55,7 +55,12 @@ struct ContractArgs {
   // C, D] (used by the elementwise kernel instead of the scatter vectors).
   int64_t h_stride[4];
   int32_t h_affine;
-  int32_t pad1;
+  // Split K: ksplit workgroups per output tile, each over kslabs K slabs,
+  // writing its unscaled partial tile to work[split][h][m][n] (storage type);
+  // a reduction pass sums the splits in order and does the write-back.
+  int32_t ksplit;
+  int64_t kslabs;
+  void* work;
 };
 
 // One item of a grouped launch: the contraction plus where its tiles start in

@@ -20,7 +20,7 @@ enum TileId : int {
   TILE_64x64 = 1,  // 4 waves of 32x32
   TILE_32x32 = 2,  // 1 wave of 32x32
   TILE_128x64 = 3, // 4 waves of 64x32
-  TILE_128x128 = 4, // 8 waves of 64x32; f32, c32/c64 native; LDS-DMA kernel only
+  TILE_128x128 = 4, // f32, c32/c64 native; LDS-DMA kernel only (8 waves of 64x32; c64 16 of 32x32)
 };
 // Tile ids of the split-BF16 kernels (16x16x32 bf16 MFMA family).
 enum SplitTileId : int {
@@ -57,6 +57,9 @@ hipError_t launch_contract_fp8(const ContractArgs& a, DType t, void* panels, hip
 // The elementwise (all-batch) strategy: one thread per output element, K
 // summed serially; for tiny M x N and K with many batch entries.
 hipError_t launch_elementwise(const ContractArgs& a, DType t, hipStream_t s);
+// Split-K reduction: sums the a.ksplit partial tiles in a.work in split order
+// and writes D = op_D(alpha * sum + beta * op_C(C)).
+hipError_t launch_splitk_reduce(const ContractArgs& a, DType t, hipStream_t s);
 // D = op_D(beta * op_C(C)) over the output (empty K or alpha == 0).
 hipError_t launch_output_pass(const ContractArgs& a, DType t, hipStream_t s);
 // Zero every element a layout addresses (rank <= 8; used by the adjoints).

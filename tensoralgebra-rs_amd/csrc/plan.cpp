@@ -262,6 +262,33 @@ void configure(Plan* plan, const PlanConfig& cfg) {
   // The 128x128 tile runs on the DMA kernel only, which needs 32-bit offsets
   // (the K order is the same for both tiles: BK 16).
   if (p.tile == TILE_128x128 && !p.off32 && cfg.tile == 0) p.tile = TILE_64x64;
+
+  // Split K when fewer than 128 tiles would run (half of the 256 CUs) and
+  // every split keeps at least 8 slabs: about 256 workgroups, at most 16
+  // splits. Split tiles run on the register-staged kernel, so a planner-chosen
+  // 128x128 tile (DMA only) gives way to 64x64 here.
+  p.ksplit = 1;
+  p.kslabs = 0;
+  auto tiles_of = [&](int id) {
+    const TileCfg* tc = tile_cfg(id);
+    return ((p.M + tc->bm - 1) / tc->bm) * ((p.N + tc->bn - 1) / tc->bn) * p.H;
+  };
+  if (p.strategy == 0 && cfg.precision == PREC_STORAGE && cfg.tile == 0 && p.tile == TILE_128x128 &&
+      tiles_of(TILE_128x128) < 128 && (p.K + 15) / 16 >= 16)
+    p.tile = TILE_64x64;
+  if (p.strategy == 0 && cfg.precision == PREC_STORAGE && p.tile != TILE_128x128) {
+    const TileCfg* tc = tile_cfg(p.tile);
+    const int64_t tiles = tiles_of(p.tile);
+    const int64_t nk = (p.K + tc->bk - 1) / tc->bk;
+    if (tiles < 128 && nk >= 16) {
+      int64_t s = std::min<int64_t>((256 + tiles - 1) / tiles, nk / 8);
+      s = std::min<int64_t>(s, 16);
+      if (s >= 2) {
+        p.ksplit = (int)s;
+        p.kslabs = (nk + s - 1) / s;
+      }
+    }
+  }
   p.ak32.clear();
   p.bk32.clear();
   if (p.off32) {
@@ -415,7 +442,9 @@ std::string Plan::kernel_name() const {
   const bool fallback = tile == TILE_128x128 && !dma_supported(off32, prob.dtype, cfg.complex_method, tile);
   const TileCfg* t = tile_cfg(fallback ? TILE_64x64 : tile);
   if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(cfg.precision == PREC_FP8 ? 128 : t->bk);
-  if (cfg.precision == PREC_STORAGE && !fallback && dma_supported(off32, prob.dtype, cfg.complex_method, tile)) s += ".dma";
+  if (cfg.precision == PREC_STORAGE && !fallback && ksplit <= 1 && dma_supported(off32, prob.dtype, cfg.complex_method, tile))
+    s += ".dma";
+  if (ksplit > 1) s += ".splitk" + std::to_string(ksplit);
   return s;
 }
 

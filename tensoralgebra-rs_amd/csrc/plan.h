@@ -46,6 +46,11 @@ struct Plan {
   // checked per call).
   bool a_kvec = false, b_kvec = false;
   bool off32 = false;  // ContractArgs::off32
+  // Split K (ContractArgs::ksplit, kslabs): 1, or the number of K ranges a
+  // tile's work is divided into when the tiles alone would leave most of the
+  // GPU idle; the partials are reduced in a fixed order.
+  int ksplit = 1;
+  int64_t kslabs = 0;
   // With off32: the K scatters of A and B as byte offsets, padded with the
   // out-of-range sentinel to a multiple of 64 entries (ContractArgs::ak32).
   std::vector<uint32_t> ak32, bk32;

@@ -134,3 +134,28 @@ def test_128x128_dma_tile_every_case(gpu, name, dtype):
     if plan.report()["strategy"] == 0:
         assert plan.report()["tile"] == 4
     check(dtype, got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
+@pytest.mark.parametrize("shape", [(40, 3000, 24, 1), (100, 700, 90, 2), (576, 576, 576, 1)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_split_k(gpu, dtype, shape):
+    """Few output tiles and a long K: the planner splits K (fixed-order
+    reduction of the partial tiles, then the write-back with alpha, beta and
+    conj) and the result matches the oracle."""
+    from harness import values
+    M, K, N, H = shape
+    a = T([M, K, H], [1, M, M * K], [0, 2, 3], conj=dtype.is_complex)
+    b = T([K, N, H], [1, K, K * N], [2, 1, 3])
+    d = T([M, N, H], [1, M, M * N], [0, 1, 3])
+    rng = np.random.default_rng(3)
+    def vals(t):
+        v = rng.uniform(-1, 1, t.storage())
+        if dtype.is_complex:
+            v = v + 1j * rng.uniform(-1, 1, t.storage())
+        return v.astype(NP[dtype])
+    got, want, plan = run(dtype, a, vals(a), b, vals(b), d, np.zeros(d.storage(), NP[dtype]),
+                          alpha=1.5, beta=-0.5, c=d, cv=vals(d))
+    assert ".splitk" in plan.report()["kernel"]
+    check(dtype, got, want)
