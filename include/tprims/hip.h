@@ -48,7 +48,9 @@ typedef struct {
   int32_t complex_method; /* TPRIMS_CPLX_* */
   int32_t precision;      /* TPRIMS_PREC_* */
   int32_t tile;           /* 0: chosen by the planner's tuner; else a tile id (see report) */
-  int32_t reserved[5];
+  int32_t split_k;        /* 0: chosen by the planner; 1: never split K; n > 1: n K ranges
+                             (storage precision, register-staged tiles; reduced in order) */
+  int32_t reserved[4];
 } tprims_plan_config;
 
 /* TAPP_create_tensor_product with a configuration (NULL = defaults). A prec of

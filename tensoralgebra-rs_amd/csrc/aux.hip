@@ -109,8 +109,20 @@ __global__ void k_splitk_reduce(ContractArgs p) {
   const int64_t per = p.M * p.N, total = per * p.H;
   const S* W = (const S*)p.work;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t n = e % p.N, r = e / p.N;
-    const int64_t m = r % p.M, h = r / p.M;
+    // Element e of one split's workspace: [h][n][m] when D is m-major, else
+    // [h][m][n] (the layout the partial tiles were written in).
+    int64_t m, n, h;
+    if (p.d_mmajor) {
+      m = e % p.M;
+      const int64_t r = e / p.M;
+      n = r % p.N;
+      h = r / p.N;
+    } else {
+      n = e % p.N;
+      const int64_t r = e / p.N;
+      m = r % p.M;
+      h = r / p.M;
+    }
     T sr = 0, si = 0;
     for (int s = 0; s < p.ksplit; s++) {
       const S v = W[(int64_t)s * total + e];

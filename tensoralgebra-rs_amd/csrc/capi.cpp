@@ -116,6 +116,7 @@ int create_product(TAPP_tensor_product* plan_out, TAPP_handle handle, TAPP_eleme
       cfg.complex_method = cfgp->complex_method;
       cfg.precision = cfgp->precision;
       cfg.tile = cfgp->tile;
+      cfg.split_k = cfgp->split_k;
     }
     if (prec == TAPP_BF16BF16_ACCUM_F32 && single) {
       if (cfg.precision == PREC_STORAGE) cfg.precision = PREC_SPLIT_BF16_3;

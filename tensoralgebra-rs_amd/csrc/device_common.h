@@ -104,8 +104,10 @@ __device__ inline void decode_tile(int64_t bid, int64_t nwg, int tiles_m, int ti
 }
 
 // Writes a staged BM x BN partial tile of split `split` (Dre/Dim, row pitch
-// LDD, in LDS) unscaled to the split-K workspace work[split][h][m][n], rows
-// past M and columns past N skipped; lanes run along n.
+// LDD, in LDS) unscaled to the split-K workspace, rows past M and columns past
+// N skipped. The workspace follows D's contiguous axis so that the reduction
+// reads and writes along the same axis: work[split][h][n][m] (lanes along m)
+// when D is m-major, work[split][h][m][n] (lanes along n) otherwise.
 template <typename T, bool CPLX, int BM, int BN, int NT, int LDD>
 __device__ inline void write_partial(const ContractArgs& p, int split, int64_t h, int64_t m0, int64_t n0, const T* Dre,
                                      const T* Dim) {
@@ -116,9 +118,10 @@ __device__ inline void write_partial(const ContractArgs& p, int split, int64_t h
 #pragma unroll 4
   for (int t = 0; t < EPI; t++) {
     const int e = threadIdx.x + t * NT;
-    const int ml = e / BN, nl = e % BN;
+    const int ml = p.d_mmajor ? e % BM : e / BN, nl = p.d_mmajor ? e / BM : e % BN;
     const int64_t m = m0 + ml, n = n0 + nl;
-    if (m < p.M && n < p.N) W[m * p.N + n] = St::make(Dre[ml * LDD + nl], CPLX ? Dim[ml * LDD + nl] : T(0));
+    const int64_t w = p.d_mmajor ? n * p.M + m : m * p.N + n;
+    if (m < p.M && n < p.N) W[w] = St::make(Dre[ml * LDD + nl], CPLX ? Dim[ml * LDD + nl] : T(0));
   }
 }
 

@@ -264,29 +264,31 @@ void configure(Plan* plan, const PlanConfig& cfg) {
   if (p.tile == TILE_128x128 && !p.off32 && cfg.tile == 0) p.tile = TILE_64x64;
 
   // Split K when fewer than 128 tiles would run (half of the 256 CUs) and
-  // every split keeps at least 8 slabs: about 256 workgroups, at most 16
-  // splits. Split tiles run on the register-staged kernel, so a planner-chosen
-  // 128x128 tile (DMA only) gives way to 64x64 here.
+  // every split keeps at least 6 slabs: about 512 workgroups (two per CU), at
+  // most 16 splits (f64 576^3: 81 tiles x 6 splits measured fastest of 1..18).
+  // Split tiles run on the register-staged kernel, so a planner-chosen 128x128
+  // tile (DMA only) gives way to 64x64 here. cfg.split_k = 1 keeps one K
+  // range, n > 1 asks for n splits (at most one per slab).
   p.ksplit = 1;
   p.kslabs = 0;
   auto tiles_of = [&](int id) {
     const TileCfg* tc = tile_cfg(id);
     return ((p.M + tc->bm - 1) / tc->bm) * ((p.N + tc->bn - 1) / tc->bn) * p.H;
   };
+  if (cfg.split_k < 0 || cfg.split_k > 64) fail(ERR_INVALID_ARGUMENT, "split_k must be in [0, 64]");
   if (p.strategy == 0 && cfg.precision == PREC_STORAGE && cfg.tile == 0 && p.tile == TILE_128x128 &&
-      tiles_of(TILE_128x128) < 128 && (p.K + 15) / 16 >= 16)
+      (cfg.split_k > 1 || (cfg.split_k == 0 && tiles_of(TILE_128x128) < 128 && (p.K + 15) / 16 >= 16)))
     p.tile = TILE_64x64;
-  if (p.strategy == 0 && cfg.precision == PREC_STORAGE && p.tile != TILE_128x128) {
+  if (p.strategy == 0 && cfg.precision == PREC_STORAGE && p.tile != TILE_128x128 && cfg.split_k != 1) {
     const TileCfg* tc = tile_cfg(p.tile);
     const int64_t tiles = tiles_of(p.tile);
     const int64_t nk = (p.K + tc->bk - 1) / tc->bk;
-    if (tiles < 128 && nk >= 16) {
-      int64_t s = std::min<int64_t>((256 + tiles - 1) / tiles, nk / 8);
-      s = std::min<int64_t>(s, 16);
-      if (s >= 2) {
-        p.ksplit = (int)s;
-        p.kslabs = (nk + s - 1) / s;
-      }
+    int64_t s = 1;
+    if (cfg.split_k > 1) s = std::min<int64_t>(cfg.split_k, nk);
+    else if (tiles < 128 && nk >= 16) s = std::min<int64_t>(std::min<int64_t>((512 + tiles - 1) / tiles, nk / 6), 16);
+    if (s >= 2) {
+      p.ksplit = (int)s;
+      p.kslabs = (nk + s - 1) / s;
     }
   }
   p.ak32.clear();

@@ -26,6 +26,7 @@ struct PlanConfig {
   int32_t complex_method = CPLX_NATIVE;
   int32_t precision = PREC_STORAGE;
   int32_t tile = 0;  // 0: chosen by the planner
+  int32_t split_k = 0;  // 0: chosen by the planner, 1: never, n > 1: n K ranges
 };
 
 // Scatter vectors of a plan resident on one device (one allocation).

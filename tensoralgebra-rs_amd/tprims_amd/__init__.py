@@ -238,14 +238,15 @@ class Problem:
 class PlanConfig:
     """Plan configuration (crates/tprims-contract/src/plan/config.rs:101), GPU knobs."""
 
-    def __init__(self, complex_method=ComplexMethod.NATIVE, precision=Precision.STORAGE, tile=0):
+    def __init__(self, complex_method=ComplexMethod.NATIVE, precision=Precision.STORAGE, tile=0, split_k=0):
         self.complex_method = int(complex_method)
         self.precision = int(precision)
         self.tile = int(tile)
+        self.split_k = int(split_k)  # 0 planner's choice, 1 never, n > 1 that many K ranges
 
     def to_c(self):
         c = _capi.PlanConfigC()
-        c.complex_method, c.precision, c.tile = self.complex_method, self.precision, self.tile
+        c.complex_method, c.precision, c.tile, c.split_k = self.complex_method, self.precision, self.tile, self.split_k
         return c
 
 

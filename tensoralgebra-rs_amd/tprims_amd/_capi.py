@@ -45,7 +45,7 @@ vp = ctypes.c_void_p
 
 class PlanConfigC(ctypes.Structure):
     _fields_ = [("complex_method", ctypes.c_int32), ("precision", ctypes.c_int32), ("tile", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("split_k", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
 class PlanReportC(ctypes.Structure):

@@ -312,3 +312,18 @@ def test_planner_picks_elementwise_for_all_batch_products():
     gemm = make_plan(tp.DType.F32, T([64, 64], [1, 64], [0, 2]), T([64, 64], [1, 64], [2, 1]),
                      T([64, 64], [1, 64], [0, 1])).report()
     assert gemm["strategy"] == 0
+
+
+def test_split_k_choice_and_override():
+    """Few output tiles with a long K split K (planner's choice); split_k = 1
+    keeps one K range, n > 1 asks for n; out-of-range values are refused."""
+    import tprims_amd as tp
+    from harness import T, make_plan
+    e = 24
+    sp = [1, e, e * e, e ** 3]
+    args = (T([e] * 4, sp, [0, 4, 1, 5]), T([e] * 4, sp, [5, 2, 4, 3]), T([e] * 4, sp, [0, 1, 2, 3]))
+    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.64x64x16.splitk6"
+    assert ".splitk" not in make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=1)).report()["kernel"]
+    assert make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=3)).report()["kernel"].endswith(".splitk3")
+    with pytest.raises(tp.TprimsError):
+        make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=-1))
