@@ -607,7 +607,7 @@ hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_
     // The DMA kernel runs one tile per workgroup, within the launch limit.
     const TileCfg* c = tile_cfg(tile_id);
     const int64_t nwg = ((a.M + c->bm - 1) / c->bm) * ((a.N + c->bn - 1) / c->bn) * a.H;
-    if (nwg <= max_grid(c->threads)) return launch_contract_dma(a, t, tile_id, s);
+    if (nwg <= max_grid(c->threads)) return launch_contract_dma(a, t, method, tile_id, s);
   }
   // The DMA-only tile falls back to the register-staged 64x64 kernel.
   if (tile_id == TILE_128x128) tile_id = TILE_64x64;

@@ -1,12 +1,13 @@
 // Storage-precision contraction kernels whose gathers are LDS DMA: every
 // lane's buffer load writes its element straight into LDS (buffer_load ... lds),
 // so the slabs never pass through registers and no VALU or LDS store
-// instruction is spent on them. Selected for f32 and for c64 with the native
-// schedule when every gather offset fits the 32-bit buffer range
+// instruction is spent on them. Selected for f32, c32 and c64 (native, 1m and
+// 3m schedules) when every gather offset fits the 32-bit buffer range
 // (ContractArgs::off32). Other cases run the register-staged kernels of
 // contract.hip, with the same
 // semantics (crates/tprims-kernel/src/pack/pack.rs:57-265 packing,
-// kernels/reference/scalar.rs:31,64 microkernels, pack/writeback.rs:114-495
+// kernels/reference/scalar.rs:31 real, :64 planar, :110 1m, :136 3m
+// microkernels, pack/writeback.rs:114-495
 // write-back, driver/tile.rs:142 K accumulation).
 //
 // A DMA instruction writes lane i's element at LDS slot i of a 64-element
@@ -51,10 +52,11 @@ __device__ inline void dma_load(__amdgpu_buffer_rsrc_t r, void* dst, uint32_t of
 #endif
 }
 
-template <typename T, bool CPLX_, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_>
+template <typename T, bool CPLX_, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_, int METH_>
 struct DCfg {
   typedef T Tv;
   static constexpr bool CPLX = CPLX_, AKM = AKM_, BKM = BKM_;
+  static constexpr int METH = CPLX_ ? METH_ : CPLX_NATIVE;
   static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
   static constexpr int WAVES_M = BM / WM;
   static constexpr int NW = (BM / WM) * (BN / WN);
@@ -78,9 +80,14 @@ struct DCfg {
   static constexpr int A_INSTR = BM * BK / 64, B_INSTR = BN * BK / 64;
   static constexpr int JA = A_INSTR / NW, JB = B_INSTR / NW;  // DMA instructions per wave per slab
   static constexpr int MI = WM / 16, NI = WN / 16;
-  // Accumulators: real 1; c64 2 (re, im; the f64 MFMA negates -im*im for
-  // free); c32 3 (re*re, im*im subtracted at the end, cross).
-  static constexpr int ACCS = !CPLX ? 1 : (PLANAR ? 3 : 2);
+  // 1m runs the real GEMM [Dre; Dim] = [[Ar, -Ai], [Ai, Ar]] [Br; Bi]: a
+  // wave's WM complex rows are 2 WM real rows (re and im of a complex row
+  // adjacent), so it holds MR = 2 MI row blocks of one accumulator.
+  static constexpr int MR = METH == CPLX_1M ? 2 * MI : MI;
+  // Accumulators: real and 1m 1; c64 native 2 (re, im; the f64 MFMA negates
+  // -im*im for free); c32 native 3 (re*re, im*im subtracted at the end,
+  // cross); 3m 3 (re*re, im*im, (re+im)*(re+im)).
+  static constexpr int ACCS = !CPLX || METH == CPLX_1M ? 1 : (PLANAR || METH == CPLX_3M ? 3 : 2);
   static constexpr int LDD = BN + 1;
   static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * EW;
   // The finished tile is staged whole when it fits in the main loop's LDS,
@@ -193,11 +200,11 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
     }
   };
 
-  Acc acc[C::ACCS][C::MI][C::NI];
+  Acc acc[C::ACCS][C::MR][C::NI];
 #pragma unroll
   for (int s = 0; s < C::ACCS; s++)
 #pragma unroll
-    for (int i = 0; i < C::MI; i++)
+    for (int i = 0; i < C::MR; i++)
 #pragma unroll
       for (int j = 0; j < C::NI; j++) acc[s][i][j] = Acc{};
 
@@ -225,6 +232,36 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
         for (int i = 0; i < C::MI; i++)
 #pragma unroll
           for (int j = 0; j < C::NI; j++) acc[0][i][j] = MM::run(a[i], b[j], acc[0][i][j]);
+      } else if constexpr (C::METH == CPLX_1M) {
+        // Two real k steps of 4 (2 complex k each). Lane (lr, lk) holds
+        // expanded A at real row lr (complex row lr / 2, part rp = lr & 1)
+        // and real k lk (complex k lk / 2, part kp = lk & 1): the re part of A
+        // when rp == kp, else the im part, negated in the re row (-Ai); and
+        // B's part kp at column lr.
+        const int rp = lr & 1, kp = lk & 1, part = rp ^ kp;
+        const uint32_t sa = (rp == 0 && kp == 1 ? 0x80000000u : 0u) ^ (part ? ca : 0u);
+        const uint32_t sb = kp ? cb : 0u;
+        const T* Af = (const T*)As;
+        const T* Bf = (const T*)Bs;
+#pragma unroll
+        for (int sub = 0; sub < 2; sub++) {
+          const int kc = kk + 2 * sub + (lk >> 1);
+          T a[C::MR], b[C::NI];
+#pragma unroll
+          for (int i = 0; i < C::MR; i++) {
+            const int x = C::template slot<C::AKM, BM>(wm + 8 * i + (lr >> 1), kc);
+            a[i] = flip(C::PLANAR ? Af[part * C::PLANE + x] : Af[2 * x + part], sa);
+          }
+#pragma unroll
+          for (int j = 0; j < C::NI; j++) {
+            const int x = C::template slot<C::BKM, BN>(wn + 16 * j + lr, kc);
+            b[j] = flip(C::PLANAR ? Bf[kp * C::PLANE + x] : Bf[2 * x + kp], sb);
+          }
+#pragma unroll
+          for (int i = 0; i < C::MR; i++)
+#pragma unroll
+            for (int j = 0; j < C::NI; j++) acc[0][i][j] = MM::run(a[i], b[j], acc[0][i][j]);
+        }
       } else {
         T ar[C::MI], ai[C::MI], br[C::NI], bi[C::NI];
 #pragma unroll
@@ -250,6 +287,24 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
             br[j] = v.re;
             bi[j] = flip(v.im, cb);
           }
+        }
+        if constexpr (C::METH == CPLX_3M) {
+          // acc0 += ar br, acc1 += ai bi, acc2 += (ar + ai)(br + bi); the
+          // epilogue forms re = acc0 - acc1, im = acc2 - acc0 - acc1.
+          T as[C::MI], bs[C::NI];
+#pragma unroll
+          for (int i = 0; i < C::MI; i++) as[i] = ar[i] + ai[i];
+#pragma unroll
+          for (int j = 0; j < C::NI; j++) bs[j] = br[j] + bi[j];
+#pragma unroll
+          for (int i = 0; i < C::MI; i++)
+#pragma unroll
+            for (int j = 0; j < C::NI; j++) {
+              acc[0][i][j] = MM::run(ar[i], br[j], acc[0][i][j]);
+              acc[1][i][j] = MM::run(ai[i], bi[j], acc[1][i][j]);
+              acc[2][i][j] = MM::run(as[i], bs[j], acc[2][i][j]);
+            }
+          continue;
         }
         // c64: acc0 += ar br - ai bi, acc1 += ar bi + ai br. c32: acc0 += ar br,
         // acc1 += ai bi (subtracted at the end), acc2 += ar bi + ai br.
@@ -302,13 +357,22 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
     if (c > 0) __syncthreads();  // the previous chunk's write-back has read the staging rows
     if (wm / ER == c) {
 #pragma unroll
-      for (int i = 0; i < C::MI; i++)
+      for (int i = 0; i < C::MR; i++)
 #pragma unroll
         for (int j = 0; j < C::NI; j++)
 #pragma unroll
           for (int q = 0; q < 4; q++) {
-            const int row = wm - c * ER + 16 * i + MM::row(lk, q), col = wn + 16 * j + lr;
-            if constexpr (C::PLANAR) {
+            const int col = wn + 16 * j + lr;
+            if constexpr (C::METH == CPLX_1M) {
+              const int rr = 16 * i + MM::row(lk, q), row = wm - c * ER + rr / 2;
+              (rr & 1 ? Dim : Dre)[row * C::LDD + col] = acc[0][i][j][q];
+              continue;
+            }
+            const int row = wm - c * ER + 16 * i + MM::row(lk, q);
+            if constexpr (C::METH == CPLX_3M) {
+              Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
+              Dim[row * C::LDD + col] = acc[2][i][j][q] - acc[0][i][j][q] - acc[1][i][j][q];
+            } else if constexpr (C::PLANAR) {
               Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
               Dim[row * C::LDD + col] = acc[2][i][j][q];
             } else {
@@ -334,14 +398,33 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN>
+template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH = CPLX_NATIVE>
 hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
   if (a.a_kmajor) {
-    if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true>>(a, s);
-    return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false>>(a, s);
+    if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH>>(a, s);
+    return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH>>(a, s);
   }
-  if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true>>(a, s);
-  return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false>>(a, s);
+  if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH>>(a, s);
+  return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH>>(a, s);
+}
+
+// One complex method's tiles (c64: 64x64, and 128x128 of 16 waves for native,
+// 8 waves of 64x32 for 1m and 3m, whose accumulators do not fit the 128
+// registers a lane of a 16-wave workgroup has; c32: 64x64, 128x64 and 128x128
+// of 8 waves).
+template <typename T, int METH>
+hipError_t launch_complex(const ContractArgs& a, int tile_id, hipStream_t s) {
+  if constexpr (sizeof(T) == 8) {
+    if (tile_id == TILE_128x128) {
+      if constexpr (METH == CPLX_NATIVE) return launch_majors<T, true, 128, 128, 16, 32, 32, METH>(a, s);
+      else return launch_majors<T, true, 128, 128, 16, 64, 32, METH>(a, s);
+    }
+    return launch_majors<T, true, 64, 64, 16, 32, 32, METH>(a, s);
+  } else {
+    if (tile_id == TILE_128x128) return launch_majors<T, true, 128, 128, 16, 64, 32, METH>(a, s);
+    if (tile_id == TILE_128x64) return launch_majors<T, true, 128, 64, 16, 64, 32, METH>(a, s);
+    return launch_majors<T, true, 64, 64, 16, 32, 32, METH>(a, s);
+  }
 }
 
 }  // namespace
@@ -355,21 +438,21 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
 // waves of 64x32).
 bool dma_supported(bool off32, DType t, int method, int tile_id) {
   if (!off32) return false;
-  if (t == DType::C64) return method == CPLX_NATIVE && (tile_id == TILE_64x64 || tile_id == TILE_128x128);
-  if (t == DType::C32)
-    return method == CPLX_NATIVE && (tile_id == TILE_64x64 || tile_id == TILE_128x64 || tile_id == TILE_128x128);
+  if (t == DType::C64) return tile_id == TILE_64x64 || tile_id == TILE_128x128;
+  if (t == DType::C32) return tile_id == TILE_64x64 || tile_id == TILE_128x64 || tile_id == TILE_128x128;
   return t == DType::F32 && (tile_id == TILE_128x128 || tile_id == TILE_128x64 || tile_id == TILE_64x64);
 }
 
-hipError_t launch_contract_dma(const ContractArgs& a, DType t, int tile_id, hipStream_t s) {
+hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s) {
   if (t == DType::C64) {
-    if (tile_id == TILE_128x128) return launch_majors<double, true, 128, 128, 16, 32, 32>(a, s);
-    return launch_majors<double, true, 64, 64, 16, 32, 32>(a, s);
+    if (method == CPLX_3M) return launch_complex<double, CPLX_3M>(a, tile_id, s);
+    if (method == CPLX_1M) return launch_complex<double, CPLX_1M>(a, tile_id, s);
+    return launch_complex<double, CPLX_NATIVE>(a, tile_id, s);
   }
   if (t == DType::C32) {
-    if (tile_id == TILE_128x128) return launch_majors<float, true, 128, 128, 16, 64, 32>(a, s);
-    if (tile_id == TILE_128x64) return launch_majors<float, true, 128, 64, 16, 64, 32>(a, s);
-    return launch_majors<float, true, 64, 64, 16, 32, 32>(a, s);
+    if (method == CPLX_3M) return launch_complex<float, CPLX_3M>(a, tile_id, s);
+    if (method == CPLX_1M) return launch_complex<float, CPLX_1M>(a, tile_id, s);
+    return launch_complex<float, CPLX_NATIVE>(a, tile_id, s);
   }
   if (t != DType::F32) return hipErrorInvalidValue;
   if (tile_id == TILE_128x128) return launch_majors<float, false, 128, 128, 16, 64, 32>(a, s);

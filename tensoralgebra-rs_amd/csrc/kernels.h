@@ -42,7 +42,7 @@ hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_
 // The LDS-DMA variant of the storage kernels (f32, c64 native; 32-bit
 // gather offsets), which launch_contract picks whenever it applies.
 bool dma_supported(bool off32, DType t, int method, int tile_id);
-hipError_t launch_contract_dma(const ContractArgs& a, DType t, int tile_id, hipStream_t s);
+hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s);
 // Split-BF16 for f32/c32 storage (nterms 2 or 3): A and B are split once into
 // bf16 fragment panels (device scratch of split_panel_bytes), then multiplied.
 size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id);

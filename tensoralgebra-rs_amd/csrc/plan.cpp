@@ -118,10 +118,10 @@ bool tile_allowed(int id, DType t, int prec, int method) {
     return id == STILE_64x64 || (id == STILE_128x64 && is_complex(t)) ||
            ((id == STILE_128x128 || id == STILE_256x128) && !is_complex(t));
   if (id >= 10) return false;
-  // c64 keeps the 64x64 register-staged tile and gets the 128x128 DMA tile
-  // instead of 128x64; 128x128 exists for f32 and the native complex schedule.
+  // c64 keeps the 64x64 tile and gets the 128x128 DMA tile instead of
+  // 128x64; 128x128 exists for f32 and every complex schedule.
   if (id == TILE_128x64 && t == DType::C64) return false;
-  if (id == TILE_128x128) return t == DType::F32 || (is_complex(t) && method == CPLX_NATIVE);
+  if (id == TILE_128x128) return t == DType::F32 || is_complex(t);
   return tile_cfg(id) != nullptr;
 }
 

@@ -122,18 +122,46 @@ def test_elementwise_strategy(gpu, dtype, layout):
     check(dtype, got, want)
 
 
+def _dma_variants():
+    return [(tp.DType.F32, tp.ComplexMethod.NATIVE)] + [(dt, m) for dt in (tp.DType.C32, tp.DType.C64)
+                                                         for m in METHODS]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.C32, tp.DType.C64], ids=lambda d: d.name)
+@pytest.mark.parametrize("dtype,method", _dma_variants(), ids=lambda v: str(getattr(v, "name", v)))
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_128x128_dma_tile_every_case(gpu, name, dtype):
-    """The 128x128 tile (8 waves of 64x32 on the LDS-DMA kernel; the
-    register-staged 64x64 kernel when an operand needs 64-bit offsets) on
-    every conformance case."""
+def test_128x128_dma_tile_every_case(gpu, name, dtype, method):
+    """The 128x128 tile on the LDS-DMA kernel (the register-staged 64x64
+    kernel when an operand needs 64-bit offsets) on every conformance case,
+    with each complex schedule: native, 1m (the real GEMM of the expanded A)
+    and 3m (three real products)."""
     kw = CASES[name](dtype)
-    got, want, plan = run(dtype, config=tp.PlanConfig(tile=4), **kw)
-    if plan.report()["strategy"] == 0:
-        assert plan.report()["tile"] == 4
-    check(dtype, got, want)
+    got, want, plan = run(dtype, config=tp.PlanConfig(tile=4, complex_method=method), **kw)
+    rep = plan.report()
+    if rep["strategy"] == 0:
+        assert rep["tile"] == 4
+    exact = name in ("alpha_zero", "empty_k") and method != tp.ComplexMethod.THREE_M
+    check(dtype, got, want, method, exact=exact)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,method", _dma_variants()[1:], ids=lambda v: str(getattr(v, "name", v)))
+@pytest.mark.parametrize("tile", [1, 3, 4])
+def test_dma_complex_schedules_on_large_k(gpu, dtype, method, tile):
+    """Every complex schedule on the DMA tiles (64x64, 128x64, 128x128) over a
+    long K with conj(A), alpha, beta and a padded (non-unit stride) operand."""
+    if tile == 3 and dtype == tp.DType.C64:
+        pytest.skip("tile 3 is not offered for c64")
+    M, K, N = 150, 333, 140
+    a = T([K, M], [2, 2 * K], [2, 0], conj=True)  # every 2nd element, K-major
+    b = T([N, K], [1, N], [1, 2])                  # N-major
+    d = T([M, N], [1, M], [0, 1])
+    av, bv, cv = values(dtype, a, 41), values(dtype, b, 42), values(dtype, d, 43)
+    got, want, plan = run(dtype, a, av, b, bv, d, np.zeros(d.storage(), NP[dtype]), alpha=complex(0.75, -0.5),
+                          beta=complex(-0.5, 0.25), c=d, cv=cv,
+                          config=tp.PlanConfig(tile=tile, complex_method=method, split_k=1))
+    assert plan.report()["kernel"].endswith(".dma"), plan.report()["kernel"]
+    check(dtype, got, want, method)
 
 
 @pytest.mark.gpu
