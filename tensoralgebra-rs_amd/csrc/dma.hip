@@ -52,10 +52,13 @@ __device__ inline void dma_load(__amdgpu_buffer_rsrc_t r, void* dst, uint32_t of
 #endif
 }
 
-template <typename T, bool CPLX_, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_, int METH_>
+// PLR_: read each k step's fragments while the previous step's MFMAs run
+// (two fragment sets in registers); off where the accumulators leave no room.
+template <typename T, bool CPLX_, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_, int METH_,
+          bool PLR_ = true>
 struct DCfg {
   typedef T Tv;
-  static constexpr bool CPLX = CPLX_, AKM = AKM_, BKM = BKM_;
+  static constexpr bool CPLX = CPLX_, AKM = AKM_, BKM = BKM_, PLR = PLR_;
   static constexpr int METH = CPLX_ ? METH_ : CPLX_NATIVE;
   static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
   static constexpr int WAVES_M = BM / WM;
@@ -101,7 +104,7 @@ struct DCfg {
   static_assert(BM % 64 == 0 || AKM, "M-major rows are whole DMA instructions");
   static_assert(BN % 64 == 0 || BKM, "N-major rows are whole DMA instructions");
   static_assert(A_INSTR % NW == 0 && B_INSTR % NW == 0, "whole DMA instructions per wave");
-  static_assert(BK % 4 == 0 && WM % 16 == 0 && WN % 16 == 0, "MFMA shape");
+  static_assert(BK % 8 == 0 && WM % 16 == 0 && WN % 16 == 0, "MFMA shape; an even number of k steps per slab");
   static_assert(LDS <= 160 * 1024, "LDS budget");
 
   // LDS slot of element (row r, k) of a stage's A (rows BM) or B (rows BN) part.
@@ -217,117 +220,129 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
       return __builtin_bit_cast(T, __builtin_bit_cast(uint32_t, x) ^ sign);
     }
   };
-  auto compute = [&](int st) {
+  // One k step as the values a lane feeds its MFMAs, read from LDS ahead of
+  // the MFMAs that use them: 4 k for real (MI + NI values), native and 3m
+  // ((re, im) pairs with conjugation applied); 2 complex k for 1m (one real
+  // k step of 4 of the expanded A, MR values, and of B, NI values).
+  constexpr int KSTEP = C::METH == CPLX_1M ? 2 : 4;
+  constexpr int FA = !C::CPLX || C::METH == CPLX_1M ? C::MR : 2 * C::MI;
+  constexpr int FB = !C::CPLX || C::METH == CPLX_1M ? C::NI : 2 * C::NI;
+  struct Frag {
+    T a[FA];
+    T b[FB];
+  };
+  auto load_frag = [&](int st, int kk, Frag& f) {
     const W* As = lds + st * C::STAGE;
     const W* Bs = As + C::A_ELEMS;
+    if constexpr (!C::CPLX) {
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      if constexpr (!C::CPLX) {
-        T a[C::MI], b[C::NI];
+      for (int i = 0; i < C::MI; i++) f.a[i] = As[C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk)];
 #pragma unroll
-        for (int i = 0; i < C::MI; i++) a[i] = As[C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk)];
+      for (int j = 0; j < C::NI; j++) f.b[j] = Bs[C::template slot<C::BKM, BN>(wn + 16 * j + lr, kk + lk)];
+    } else if constexpr (C::METH == CPLX_1M) {
+      // Lane (lr, lk) holds expanded A at real row lr (complex row lr / 2,
+      // part rp = lr & 1) and real k lk (complex k lk / 2, part kp = lk & 1):
+      // the re part of A when rp == kp, else the im part, negated in the re
+      // row (-Ai); and B's part kp at column lr.
+      const int rp = lr & 1, kp = lk & 1, part = rp ^ kp;
+      const uint32_t sa = (rp == 0 && kp == 1 ? 0x80000000u : 0u) ^ (part ? ca : 0u);
+      const uint32_t sb = kp ? cb : 0u;
+      const T* Af = (const T*)As;
+      const T* Bf = (const T*)Bs;
+      const int kc = kk + (lk >> 1);
 #pragma unroll
-        for (int j = 0; j < C::NI; j++) b[j] = Bs[C::template slot<C::BKM, BN>(wn + 16 * j + lr, kk + lk)];
+      for (int i = 0; i < C::MR; i++) {
+        const int x = C::template slot<C::AKM, BM>(wm + 8 * i + (lr >> 1), kc);
+        f.a[i] = flip(C::PLANAR ? Af[part * C::PLANE + x] : Af[2 * x + part], sa);
+      }
 #pragma unroll
-        for (int i = 0; i < C::MI; i++)
+      for (int j = 0; j < C::NI; j++) {
+        const int x = C::template slot<C::BKM, BN>(wn + 16 * j + lr, kc);
+        f.b[j] = flip(C::PLANAR ? Bf[kp * C::PLANE + x] : Bf[2 * x + kp], sb);
+      }
+    } else {
 #pragma unroll
-          for (int j = 0; j < C::NI; j++) acc[0][i][j] = MM::run(a[i], b[j], acc[0][i][j]);
-      } else if constexpr (C::METH == CPLX_1M) {
-        // Two real k steps of 4 (2 complex k each). Lane (lr, lk) holds
-        // expanded A at real row lr (complex row lr / 2, part rp = lr & 1)
-        // and real k lk (complex k lk / 2, part kp = lk & 1): the re part of A
-        // when rp == kp, else the im part, negated in the re row (-Ai); and
-        // B's part kp at column lr.
-        const int rp = lr & 1, kp = lk & 1, part = rp ^ kp;
-        const uint32_t sa = (rp == 0 && kp == 1 ? 0x80000000u : 0u) ^ (part ? ca : 0u);
-        const uint32_t sb = kp ? cb : 0u;
-        const T* Af = (const T*)As;
-        const T* Bf = (const T*)Bs;
-#pragma unroll
-        for (int sub = 0; sub < 2; sub++) {
-          const int kc = kk + 2 * sub + (lk >> 1);
-          T a[C::MR], b[C::NI];
-#pragma unroll
-          for (int i = 0; i < C::MR; i++) {
-            const int x = C::template slot<C::AKM, BM>(wm + 8 * i + (lr >> 1), kc);
-            a[i] = flip(C::PLANAR ? Af[part * C::PLANE + x] : Af[2 * x + part], sa);
-          }
-#pragma unroll
-          for (int j = 0; j < C::NI; j++) {
-            const int x = C::template slot<C::BKM, BN>(wn + 16 * j + lr, kc);
-            b[j] = flip(C::PLANAR ? Bf[kp * C::PLANE + x] : Bf[2 * x + kp], sb);
-          }
-#pragma unroll
-          for (int i = 0; i < C::MR; i++)
-#pragma unroll
-            for (int j = 0; j < C::NI; j++) acc[0][i][j] = MM::run(a[i], b[j], acc[0][i][j]);
+      for (int i = 0; i < C::MI; i++) {
+        const int x = C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk);
+        if constexpr (C::PLANAR) {
+          f.a[2 * i] = As[x];
+          f.a[2 * i + 1] = flip(As[C::PLANE + x], ca);
+        } else {
+          const S v = As[x];
+          f.a[2 * i] = v.re;
+          f.a[2 * i + 1] = flip(v.im, ca);
         }
-      } else {
-        T ar[C::MI], ai[C::MI], br[C::NI], bi[C::NI];
+      }
 #pragma unroll
-        for (int i = 0; i < C::MI; i++) {
-          const int x = C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk);
-          if constexpr (C::PLANAR) {
-            ar[i] = As[x];
-            ai[i] = flip(As[C::PLANE + x], ca);
-          } else {
-            const S v = As[x];
-            ar[i] = v.re;
-            ai[i] = flip(v.im, ca);
-          }
+      for (int j = 0; j < C::NI; j++) {
+        const int x = C::template slot<C::BKM, BN>(wn + 16 * j + lr, kk + lk);
+        if constexpr (C::PLANAR) {
+          f.b[2 * j] = Bs[x];
+          f.b[2 * j + 1] = flip(Bs[C::PLANE + x], cb);
+        } else {
+          const S v = Bs[x];
+          f.b[2 * j] = v.re;
+          f.b[2 * j + 1] = flip(v.im, cb);
         }
-#pragma unroll
-        for (int j = 0; j < C::NI; j++) {
-          const int x = C::template slot<C::BKM, BN>(wn + 16 * j + lr, kk + lk);
-          if constexpr (C::PLANAR) {
-            br[j] = Bs[x];
-            bi[j] = flip(Bs[C::PLANE + x], cb);
-          } else {
-            const S v = Bs[x];
-            br[j] = v.re;
-            bi[j] = flip(v.im, cb);
-          }
-        }
-        if constexpr (C::METH == CPLX_3M) {
-          // acc0 += ar br, acc1 += ai bi, acc2 += (ar + ai)(br + bi); the
-          // epilogue forms re = acc0 - acc1, im = acc2 - acc0 - acc1.
-          T as[C::MI], bs[C::NI];
-#pragma unroll
-          for (int i = 0; i < C::MI; i++) as[i] = ar[i] + ai[i];
-#pragma unroll
-          for (int j = 0; j < C::NI; j++) bs[j] = br[j] + bi[j];
-#pragma unroll
-          for (int i = 0; i < C::MI; i++)
-#pragma unroll
-            for (int j = 0; j < C::NI; j++) {
-              acc[0][i][j] = MM::run(ar[i], br[j], acc[0][i][j]);
-              acc[1][i][j] = MM::run(ai[i], bi[j], acc[1][i][j]);
-              acc[2][i][j] = MM::run(as[i], bs[j], acc[2][i][j]);
-            }
-          continue;
-        }
-        // c64: acc0 += ar br - ai bi, acc1 += ar bi + ai br. c32: acc0 += ar br,
-        // acc1 += ai bi (subtracted at the end), acc2 += ar bi + ai br.
-        constexpr int XI = C::PLANAR ? 2 : 1;
-#pragma unroll
-        for (int i = 0; i < C::MI; i++)
-#pragma unroll
-          for (int j = 0; j < C::NI; j++) {
-            acc[0][i][j] = MM::run(ar[i], br[j], acc[0][i][j]);
-            acc[XI][i][j] = MM::run(ar[i], bi[j], acc[XI][i][j]);
-          }
-#pragma unroll
-        for (int i = 0; i < C::MI; i++)
-#pragma unroll
-          for (int j = 0; j < C::NI; j++) {
-            if constexpr (C::PLANAR) acc[1][i][j] = MM::run(ai[i], bi[j], acc[1][i][j]);
-            else acc[0][i][j] = MM::run_neg_a(ai[i], bi[j], acc[0][i][j]);
-            acc[XI][i][j] = MM::run(ai[i], br[j], acc[XI][i][j]);
-          }
       }
     }
   };
+  auto mma = [&](const Frag& f) {
+    if constexpr (!C::CPLX) {
+#pragma unroll
+      for (int i = 0; i < C::MI; i++)
+#pragma unroll
+        for (int j = 0; j < C::NI; j++) acc[0][i][j] = MM::run(f.a[i], f.b[j], acc[0][i][j]);
+    } else if constexpr (C::METH == CPLX_1M) {
+#pragma unroll
+      for (int i = 0; i < C::MR; i++)
+#pragma unroll
+        for (int j = 0; j < C::NI; j++) acc[0][i][j] = MM::run(f.a[i], f.b[j], acc[0][i][j]);
+    } else if constexpr (C::METH == CPLX_3M) {
+      // acc0 += ar br, acc1 += ai bi, acc2 += (ar + ai)(br + bi); the
+      // epilogue forms re = acc0 - acc1, im = acc2 - acc0 - acc1.
+      T as[C::MI], bs[C::NI];
+#pragma unroll
+      for (int i = 0; i < C::MI; i++) as[i] = f.a[2 * i] + f.a[2 * i + 1];
+#pragma unroll
+      for (int j = 0; j < C::NI; j++) bs[j] = f.b[2 * j] + f.b[2 * j + 1];
+#pragma unroll
+      for (int i = 0; i < C::MI; i++)
+#pragma unroll
+        for (int j = 0; j < C::NI; j++) {
+          acc[0][i][j] = MM::run(f.a[2 * i], f.b[2 * j], acc[0][i][j]);
+          acc[1][i][j] = MM::run(f.a[2 * i + 1], f.b[2 * j + 1], acc[1][i][j]);
+          acc[2][i][j] = MM::run(as[i], bs[j], acc[2][i][j]);
+        }
+    } else {
+      // c64: acc0 += ar br - ai bi, acc1 += ar bi + ai br. c32: acc0 += ar br,
+      // acc1 += ai bi (subtracted at the end), acc2 += ar bi + ai br.
+      constexpr int XI = C::PLANAR ? 2 : 1;
+#pragma unroll
+      for (int i = 0; i < C::MI; i++)
+#pragma unroll
+        for (int j = 0; j < C::NI; j++) {
+          acc[0][i][j] = MM::run(f.a[2 * i], f.b[2 * j], acc[0][i][j]);
+          acc[XI][i][j] = MM::run(f.a[2 * i], f.b[2 * j + 1], acc[XI][i][j]);
+        }
+#pragma unroll
+      for (int i = 0; i < C::MI; i++)
+#pragma unroll
+        for (int j = 0; j < C::NI; j++) {
+          if constexpr (C::PLANAR) acc[1][i][j] = MM::run(f.a[2 * i + 1], f.b[2 * j + 1], acc[1][i][j]);
+          else acc[0][i][j] = MM::run_neg_a(f.a[2 * i + 1], f.b[2 * j + 1], acc[0][i][j]);
+          acc[XI][i][j] = MM::run(f.a[2 * i + 1], f.b[2 * j], acc[XI][i][j]);
+        }
+    }
+  };
 
+  // Main loop: every k step's fragments are read from LDS while the previous
+  // step's MFMAs run. The slab barrier sits before the last step's MFMAs (its
+  // fragments are already in registers): it publishes slab kt + 1 (each
+  // wave waited for its own DMA) and retires stage kt & 1 (every wave has
+  // read it), so the DMA of slab kt + 2 and the first fragments of slab
+  // kt + 1 are issued under those MFMAs.
+  constexpr int KQ = BK / KSTEP;
   const int64_t nk = (K + BK - 1) / BK;
   uint32_t a_ko[C::JA], b_ko[C::JB];
   {
@@ -339,13 +354,42 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
   }
   __builtin_amdgcn_s_waitcnt(7 << 4);  // vmcnt(0) lgkmcnt(0)
   __syncthreads();
+  Frag f[C::PLR ? 2 : 1];
+  if constexpr (!C::PLR) {
+    for (int64_t kt = 0; kt < nk; kt++) {
+      const bool more = kt + 2 < nk;
+      const int st = (int)(kt & 1);
+      if (more) load_koff((kt + 2) * BK, a_ko, b_ko);
+#pragma unroll
+      for (int q = 0; q < KQ; q++) {
+        load_frag(st, KSTEP * q, f[0]);
+        mma(f[0]);
+      }
+      __builtin_amdgcn_s_waitcnt(7 << 4);  // this wave's DMA of slab kt + 1 (and the offsets)
+      __syncthreads();
+      if (more) issue(st, a_ko, b_ko);
+    }
+  } else {
+  load_frag(0, 0, f[0]);
   for (int64_t kt = 0; kt < nk; kt++) {
     const bool more = kt + 2 < nk;
+    const int st = (int)(kt & 1);
     if (more) load_koff((kt + 2) * BK, a_ko, b_ko);  // lands during the MFMAs
-    compute((int)(kt & 1));
-    __builtin_amdgcn_s_waitcnt(7 << 4);  // this wave's DMA of slab kt + 1 (and the offsets)
-    __syncthreads();
-    if (more) issue((int)(kt & 1), a_ko, b_ko);
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+      if (q + 1 < KQ) {
+        load_frag(st, KSTEP * (q + 1), f[(q + 1) & 1]);
+      } else {
+        __builtin_amdgcn_s_waitcnt(7 << 4);  // this wave's DMA of slab kt + 1, the offsets, its LDS reads
+        __syncthreads();
+        if (more) issue(st, a_ko, b_ko);
+        if (kt + 1 < nk) load_frag(st ^ 1, 0, f[(q + 1) & 1]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      mma(f[q & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
   }
 
   // Stage the tile chunk by chunk (Dre/Dim [EPI_ROWS][LDD]) and write it back.
@@ -398,30 +442,34 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH = CPLX_NATIVE>
+template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH = CPLX_NATIVE, bool PLR = true>
 hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
   if (a.a_kmajor) {
-    if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH>>(a, s);
-    return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH>>(a, s);
+    if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH, PLR>>(a, s);
+    return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH, PLR>>(a, s);
   }
-  if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH>>(a, s);
-  return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH>>(a, s);
+  if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH, PLR>>(a, s);
+  return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR>>(a, s);
 }
 
 // One complex method's tiles (c64: 64x64, and 128x128 of 16 waves for native,
 // 8 waves of 64x32 for 1m and 3m, whose accumulators do not fit the 128
-// registers a lane of a 16-wave workgroup has; c32: 64x64, 128x64 and 128x128
-// of 8 waves).
+// registers a lane of a 16-wave workgroup has, and then leave no room for a
+// second fragment set; c32: 64x64, 128x64 and 128x128 of 8 waves). Fragment
+// prefetch measured on the complex benchmark contraction (same GPU, same
+// run): c64 native 128x128 62.5 -> 64.3 TFLOP/s, c32 1m 128x128 83 -> 86;
+// c32 native 128x128 103 -> 95 and c64 3m 64x64 63 -> 54 (one wave per SIMD),
+// so those read each step as they go.
 template <typename T, int METH>
 hipError_t launch_complex(const ContractArgs& a, int tile_id, hipStream_t s) {
   if constexpr (sizeof(T) == 8) {
     if (tile_id == TILE_128x128) {
       if constexpr (METH == CPLX_NATIVE) return launch_majors<T, true, 128, 128, 16, 32, 32, METH>(a, s);
-      else return launch_majors<T, true, 128, 128, 16, 64, 32, METH>(a, s);
+      else return launch_majors<T, true, 128, 128, 16, 64, 32, METH, false>(a, s);
     }
-    return launch_majors<T, true, 64, 64, 16, 32, 32, METH>(a, s);
+    return launch_majors<T, true, 64, 64, 16, 32, 32, METH, METH != CPLX_3M>(a, s);
   } else {
-    if (tile_id == TILE_128x128) return launch_majors<T, true, 128, 128, 16, 64, 32, METH>(a, s);
+    if (tile_id == TILE_128x128) return launch_majors<T, true, 128, 128, 16, 64, 32, METH, METH != CPLX_NATIVE>(a, s);
     if (tile_id == TILE_128x64) return launch_majors<T, true, 128, 64, 16, 64, 32, METH>(a, s);
     return launch_majors<T, true, 64, 64, 16, 32, 32, METH>(a, s);
   }
@@ -455,7 +503,7 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int t
     return launch_complex<float, CPLX_NATIVE>(a, tile_id, s);
   }
   if (t != DType::F32) return hipErrorInvalidValue;
-  if (tile_id == TILE_128x128) return launch_majors<float, false, 128, 128, 16, 64, 32>(a, s);
+  if (tile_id == TILE_128x128) return launch_majors<float, false, 128, 128, 16, 64, 32, CPLX_NATIVE, false>(a, s);  // prefetch: 122.7 -> 121.1
   if (tile_id == TILE_128x64) return launch_majors<float, false, 128, 64, 16, 64, 32>(a, s);
   return launch_majors<float, false, 64, 64, 16, 32, 32>(a, s);
 }
