@@ -1,8 +1,8 @@
 // Storage-precision contraction kernels whose gathers are LDS DMA: every
 // lane's buffer load writes its element straight into LDS (buffer_load ... lds),
 // so the slabs never pass through registers and no VALU or LDS store
-// instruction is spent on them. Selected for f32, c32 and c64 (native, 1m and
-// 3m schedules) when every gather offset fits the 32-bit buffer range
+// instruction is spent on them. Selected for f32, f64, c32 and c64 (native, 1m
+// and 3m schedules) when every gather offset fits the 32-bit buffer range
 // (ContractArgs::off32). Other cases run the register-staged kernels of
 // contract.hip, with the same
 // semantics (crates/tprims-kernel/src/pack/pack.rs:57-265 packing,
@@ -66,9 +66,10 @@ struct DCfg {
   static constexpr int NT = NW * 64;
   typedef typename Store<T, CPLX>::S S;
   static constexpr int ES = sizeof(S);  // element bytes in memory
-  // c32 elements (8 bytes, no DMA size) arrive as two 4-byte DMAs into
-  // separate re and im planes; f32 and c64 elements as one DMA each.
-  static constexpr bool PLANAR = CPLX && sizeof(T) == 4;
+  // 8-byte elements (no DMA size) arrive as two 4-byte DMAs into separate
+  // planes: re and im for c32, the low and high words for f64; f32 and c64
+  // elements as one DMA each.
+  static constexpr bool PLANAR = ES == 8;
   static constexpr int PLANES = PLANAR ? 2 : 1;
   typedef typename std::conditional<PLANAR, float, S>::type W;  // LDS element
   static constexpr int EW = sizeof(W);  // DMA size: 4 or 16
@@ -234,7 +235,22 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
   auto load_frag = [&](int st, int kk, Frag& f) {
     const W* As = lds + st * C::STAGE;
     const W* Bs = As + C::A_ELEMS;
-    if constexpr (!C::CPLX) {
+    if constexpr (!C::CPLX && C::PLANAR) {
+      // f64 from its word planes.
+      auto join = [](W lo, W hi) {
+        return __builtin_bit_cast(T, ((uint64_t)__builtin_bit_cast(uint32_t, hi) << 32) | __builtin_bit_cast(uint32_t, lo));
+      };
+#pragma unroll
+      for (int i = 0; i < C::MI; i++) {
+        const int x = C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk);
+        f.a[i] = join(As[x], As[C::PLANE + x]);
+      }
+#pragma unroll
+      for (int j = 0; j < C::NI; j++) {
+        const int x = C::template slot<C::BKM, BN>(wn + 16 * j + lr, kk + lk);
+        f.b[j] = join(Bs[x], Bs[C::PLANE + x]);
+      }
+    } else if constexpr (!C::CPLX) {
 #pragma unroll
       for (int i = 0; i < C::MI; i++) f.a[i] = As[C::template slot<C::AKM, BM>(wm + 16 * i + lr, kk + lk)];
 #pragma unroll
@@ -416,7 +432,7 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
             if constexpr (C::METH == CPLX_3M) {
               Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
               Dim[row * C::LDD + col] = acc[2][i][j][q] - acc[0][i][j][q] - acc[1][i][j][q];
-            } else if constexpr (C::PLANAR) {
+            } else if constexpr (C::CPLX && C::PLANAR) {
               Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
               Dim[row * C::LDD + col] = acc[2][i][j][q];
             } else {
@@ -488,7 +504,7 @@ bool dma_supported(bool off32, DType t, int method, int tile_id) {
   if (!off32) return false;
   if (t == DType::C64) return tile_id == TILE_64x64 || tile_id == TILE_128x128;
   if (t == DType::C32) return tile_id == TILE_64x64 || tile_id == TILE_128x64 || tile_id == TILE_128x128;
-  return t == DType::F32 && (tile_id == TILE_128x128 || tile_id == TILE_128x64 || tile_id == TILE_64x64);
+  return tile_id == TILE_128x128 || tile_id == TILE_128x64 || tile_id == TILE_64x64;  // f32, f64
 }
 
 hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s) {
@@ -502,7 +518,11 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int t
     if (method == CPLX_1M) return launch_complex<float, CPLX_1M>(a, tile_id, s);
     return launch_complex<float, CPLX_NATIVE>(a, tile_id, s);
   }
-  if (t != DType::F32) return hipErrorInvalidValue;
+  if (t == DType::F64) {
+    if (tile_id == TILE_128x128) return launch_majors<double, false, 128, 128, 16, 64, 32>(a, s);
+    if (tile_id == TILE_128x64) return launch_majors<double, false, 128, 64, 16, 64, 32>(a, s);
+    return launch_majors<double, false, 64, 64, 16, 32, 32>(a, s);
+  }
   if (tile_id == TILE_128x128) return launch_majors<float, false, 128, 128, 16, 64, 32, CPLX_NATIVE, false>(a, s);  // prefetch: 122.7 -> 121.1
   if (tile_id == TILE_128x64) return launch_majors<float, false, 128, 64, 16, 64, 32>(a, s);
   return launch_majors<float, false, 64, 64, 16, 32, 32>(a, s);

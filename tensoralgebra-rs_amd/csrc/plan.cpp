@@ -119,9 +119,8 @@ bool tile_allowed(int id, DType t, int prec, int method) {
            ((id == STILE_128x128 || id == STILE_256x128) && !is_complex(t));
   if (id >= 10) return false;
   // c64 keeps the 64x64 tile and gets the 128x128 DMA tile instead of
-  // 128x64; 128x128 exists for f32 and every complex schedule.
+  // 128x64; 128x128 (DMA only) exists for every dtype and complex schedule.
   if (id == TILE_128x64 && t == DType::C64) return false;
-  if (id == TILE_128x128) return t == DType::F32 || is_complex(t);
   return tile_cfg(id) != nullptr;
 }
 

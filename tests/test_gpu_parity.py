@@ -85,8 +85,6 @@ def test_host_operands_are_staged(gpu, name, dtype):
 def test_every_tile(gpu, tile, dtype):
     if tile == 3 and dtype == tp.DType.C64:
         pytest.skip("tile 3 is not offered for c64")
-    if tile == 4 and dtype == tp.DType.F64:
-        pytest.skip("tile 4 (128x128, LDS-DMA) is offered for f32, c32 and c64 only")
     kw = CASES["large_k"](dtype)
     got, want, plan = run(dtype, config=tp.PlanConfig(tile=tile), **kw)
     assert plan.report()["tile"] == tile
@@ -123,8 +121,8 @@ def test_elementwise_strategy(gpu, dtype, layout):
 
 
 def _dma_variants():
-    return [(tp.DType.F32, tp.ComplexMethod.NATIVE)] + [(dt, m) for dt in (tp.DType.C32, tp.DType.C64)
-                                                         for m in METHODS]
+    real = [(tp.DType.F32, tp.ComplexMethod.NATIVE), (tp.DType.F64, tp.ComplexMethod.NATIVE)]
+    return real + [(dt, m) for dt in (tp.DType.C32, tp.DType.C64) for m in METHODS]
 
 
 @pytest.mark.gpu
@@ -145,7 +143,7 @@ def test_128x128_dma_tile_every_case(gpu, name, dtype, method):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dtype,method", _dma_variants()[1:], ids=lambda v: str(getattr(v, "name", v)))
+@pytest.mark.parametrize("dtype,method", _dma_variants()[2:], ids=lambda v: str(getattr(v, "name", v)))
 @pytest.mark.parametrize("tile", [1, 3, 4])
 def test_dma_complex_schedules_on_large_k(gpu, dtype, method, tile):
     """Every complex schedule on the DMA tiles (64x64, 128x64, 128x128) over a
