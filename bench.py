@@ -11,6 +11,7 @@ with HIP events on the executor's stream.
 The JSON line also carries, under "configs", the other benchmark
 contractions measured the same way:
   f64_24      D[a,b,c,d] = sum_{e,f} A[a,e,b,f] B[f,c,e,d], all extents 24
+  f64_gemm4096  f64 D[m,n] = sum_k A[m,k] B[k,n], 4096^3, column-major
   transform   f32 D[p,q,r,s] = sum_t A[p,q,r,t] U[t,s], p=q=r=64, t=s=512,
               rows N(0,1) * 10^U(-3,3): FP32, split-BF16 3-term, 2-term
   gemm4096    f32 4096^3, same distribution and precisions
@@ -147,6 +148,31 @@ def config_f64_24(torch, tp, np, ex, stream, steps):
     got = D.cpu().numpy().reshape([e] * 4, order="F")
     return {"workload": "f64 D[a,b,c,d]=sum_{e,f} A[a,e,b,f] B[f,c,e,d], extents 24", "ms": round(ms, 4),
             "tflops": round(flops / ms / 1e9, 3), "frac_fp64_peak": round(flops / ms / 1e9 / PEAK_TFLOPS["f64"], 4),
+            "rel_err_vs_fp64": rel_err(np, got, want), "kernel": plan.report()["kernel"]}
+
+
+def config_f64_gemm(torch, tp, np, ex, stream, steps, n=4096):
+    """f64 GEMM-shaped contraction D[m,n] = sum_k A[m,k] B[k,n], column-major."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(7)
+    A = torch.randn(n * n, dtype=torch.float64, device=dev, generator=g)
+    B = torch.randn(n * n, dtype=torch.float64, device=dev, generator=g)
+    D = torch.zeros(n * n, dtype=torch.float64, device=dev)
+    sp = lambda: tp.OperandSpec(tp.LayoutSpec([n, n], [1, n]))
+    plan = tp.Plan(tp.Problem.from_labels(tp.DType.F64, sp(), sp(), tp.CSpec.separate(sp()), sp(),
+                                          tp.Labels([0, 2], [2, 1], [0, 1], [0, 1])))
+    run = lambda: plan.execute_raw(1.0, A.data_ptr(), B.data_ptr(), 0.0, None, D.data_ptr(), ex)
+    ms = timed(torch, run, stream, max(3, steps // 2), 2)
+    rng = np.random.default_rng(8)
+    si, sj = rng.integers(0, n, 256), rng.integers(0, n, 256)
+    Ah = A.cpu().numpy().reshape(n, n)   # [k, m]
+    Bh = B.cpu().numpy().reshape(n, n)   # [n, k]
+    Dh = D.cpu().numpy().reshape(n, n)   # [n, m]
+    want = np.array([Ah[:, i] @ Bh[j, :] for i, j in zip(si, sj)])
+    got = np.array([Dh[j, i] for i, j in zip(si, sj)])
+    tf = 2 * n ** 3 / ms / 1e9
+    return {"workload": "f64 GEMM D[m,n]=sum_k A[m,k] B[k,n], %d^3, column-major" % n, "ms": round(ms, 4),
+            "tflops": round(tf, 3), "frac_fp64_peak": round(tf / PEAK_TFLOPS["f64"], 4),
             "rel_err_vs_fp64": rel_err(np, got, want), "kernel": plan.report()["kernel"]}
 
 
@@ -391,6 +417,7 @@ def main():
     if not args.no_configs:
         cfgs = {}
         for name, fn in (("f64_24", lambda: config_f64_24(torch, tp, np, ex, stream, args.steps)),
+                         ("f64_gemm4096", lambda: config_f64_gemm(torch, tp, np, ex, stream, args.steps)),
                          ("transform", lambda: config_precision(torch, tp, np, ex, stream, args.steps, "transform")),
                          ("gemm4096", lambda: config_precision(torch, tp, np, ex, stream, args.steps, "gemm")),
                          ("grouped", lambda: config_grouped(torch, tp, np, ex, stream, args.steps)),
