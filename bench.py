@@ -17,6 +17,8 @@ contractions measured the same way:
   gemm4096    f32 4096^3, same distribution and precisions
   grouped     512 MPS contractions E[a,s,b] = sum_{c,t} L[a,c] M[c,t,s,b] (c64):
               one persistent launch (autotuned tile, hipGraph) vs one launch each
+  tapp_batched  512 c64 64^3 GEMMs of one plan through TAPP_execute_batched_product
+              (one launch) vs one call per item
   vjp         D[i,k,b] = sum_{j,r} A[i,i,j,r,b] B[j,k,b] forward + dA, dB (f32, c64)
 with relative errors against FP64 evaluations of sampled outputs, and the CPU
 baseline (the C++ port of the reference's packed driver on the host cores).
@@ -261,6 +263,37 @@ def config_grouped(torch, tp, np, ex, stream, steps, n_items=512):
             "speedup_vs_per_launch": round(t_each / min(t_group, t_graph), 3), "matches_per_launch": same}
 
 
+def config_tapp_batched(torch, tp, np, ex, stream, steps, n=512, e=64):
+    """TAPP_execute_batched_product: one c64 plan (e^3 GEMM) on n items, as one
+    launch (items folded into the batch offsets) vs one call per item."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(9)
+    sp = lambda: tp.OperandSpec(tp.LayoutSpec([e, e], [1, e]))
+    plan = tp.Plan(tp.Problem.from_labels(tp.DType.C64, sp(), sp(), tp.CSpec.separate(sp()), sp(),
+                                          tp.Labels([0, 2], [2, 1], [0, 1], [0, 1])))
+    A = torch.randn(n, e * e, dtype=torch.complex128, device=dev, generator=g)
+    B = torch.randn(n, e * e, dtype=torch.complex128, device=dev, generator=g)
+    D = torch.zeros(n, e * e, dtype=torch.complex128, device=dev)
+    pa, pb, pd = ([x[i].data_ptr() for i in range(n)] for x in (A, B, D))
+    fused = lambda: plan.execute_batched_raw(1.0, pa, pb, 0.0, None, pd, ex)
+
+    def each():
+        for i in range(n):
+            plan.execute_raw(1.0, pa[i], pb[i], 0.0, None, pd[i], ex)
+
+    t_fused = timed(torch, fused, stream, steps, 2)
+    got = D.clone()
+    t_each = timed(torch, each, stream, max(2, steps // 4), 1)
+    flops = 8 * e ** 3 * n
+    i = n // 3
+    want = (A[i].view(e, e).t().cpu() @ B[i].view(e, e).t().cpu()).t().reshape(-1)
+    err = float((got[i].cpu() - want).abs().max() / want.abs().max())
+    return {"workload": "%d c64 GEMMs %d^3 through TAPP_execute_batched_product" % (n, e), "flops": flops,
+            "batched_ms": round(t_fused, 4), "batched_tflops": round(flops / t_fused / 1e9, 3),
+            "per_call_ms": round(t_each, 4), "per_call_tflops": round(flops / t_each / 1e9, 3),
+            "speedup": round(t_each / t_fused, 2), "rel_err_item": err, "kernel": plan.report()["kernel"]}
+
+
 def config_vjp(torch, tp, np, ex, stream, steps):
     dev = torch.device("cuda:0")
     out = {"workload": "D[i,k,b]=sum_{j,r} A[i,i,j,r,b] B[j,k,b], i=k=256, j=128, r=8, b=32: forward + dA, dB"}
@@ -421,6 +454,7 @@ def main():
                          ("transform", lambda: config_precision(torch, tp, np, ex, stream, args.steps, "transform")),
                          ("gemm4096", lambda: config_precision(torch, tp, np, ex, stream, args.steps, "gemm")),
                          ("grouped", lambda: config_grouped(torch, tp, np, ex, stream, args.steps)),
+                         ("tapp_batched", lambda: config_tapp_batched(torch, tp, np, ex, stream, args.steps)),
                          ("vjp", lambda: config_vjp(torch, tp, np, ex, stream, args.steps))):
             try:
                 cfgs[name] = fn()

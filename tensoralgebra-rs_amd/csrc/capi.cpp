@@ -149,6 +149,9 @@ int execute_items(Plan* p, TAPP_executor exec, const void* alpha, const void* be
     Executor* e;
     ~Release() { e->inflight--; }
   } rel{ex};
+  // A batch runs as one launch when that equals running its items in order
+  // (TAPP runs them in order, crates/tprims-capi/src/execute.rs:235).
+  if (items.size() > 1 && run_items_fused(*p, *ex, al, be, items)) return 0;
   for (auto& it : items) run_item(*p, *ex, al, be, it);
   return 0;
 }

@@ -2,6 +2,8 @@
 // of an execution, host-operand staging and the launch of the planned kernel.
 #include "executor.h"
 
+#include <algorithm>
+#include <climits>
 #include <map>
 #include <string>
 #include <vector>
@@ -69,10 +71,25 @@ char* Executor::ensure_panels(size_t bytes) {
   return panels;
 }
 
+char* Executor::ensure_batch(size_t bytes) {
+  if (bytes <= batch_cap) return batch_vecs;
+  if (batch_vecs) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    hip_check(hipFree(batch_vecs), "hipFree");
+    batch_vecs = nullptr;
+    batch_cap = 0;
+  }
+  size_t cap = bytes < ((size_t)1 << 16) ? ((size_t)1 << 16) : bytes + bytes / 8;
+  hip_check(hipMalloc((void**)&batch_vecs, cap), "hipMalloc(batch offsets)");
+  batch_cap = cap;
+  return batch_vecs;
+}
+
 Executor::~Executor() {
   if (stream_ready && stream) (void)hipStreamSynchronize(stream);
   if (scratch) (void)hipFree(scratch);
   if (panels) (void)hipFree(panels);
+  if (batch_vecs) (void)hipFree(batch_vecs);
   if (owns_stream && stream_ready && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -172,6 +189,36 @@ Where classify(const void* p, int device) {
 
 bool is_zero(const double v[2]) { return v[0] == 0.0 && v[1] == 0.0; }
 
+// The launches of one planned call on device-resident arguments: output pass,
+// elementwise strategy, FP8, split-BF16, split K (partials then the ordered
+// reduction) or the contraction kernel. ksplit <= 1 runs K in one range.
+void launch_plan(Plan& p, Executor& ex, ContractArgs& a, bool product, int ksplit, int64_t kslabs, hipStream_t s) {
+  const int64_t es = elem_size(p.prob.dtype);
+  if (!product) {
+    hip_check(launch_output_pass(a, p.prob.dtype, s), "output-pass launch");
+  } else if (p.strategy == 3) {
+    hip_check(launch_elementwise(a, p.prob.dtype, s), "elementwise launch");
+  } else if (p.cfg.precision == PREC_FP8) {
+    std::lock_guard<std::mutex> g(ex.panel_mu);
+    char* panels = ex.ensure_panels(fp8_panel_bytes(a, p.prob.dtype));
+    hip_check(launch_contract_fp8(a, p.prob.dtype, panels, s), "FP8 contraction launch");
+  } else if (p.cfg.precision == PREC_SPLIT_BF16_2 || p.cfg.precision == PREC_SPLIT_BF16_3) {
+    const int terms = p.cfg.precision == PREC_SPLIT_BF16_2 ? 2 : 3;
+    std::lock_guard<std::mutex> g(ex.panel_mu);
+    char* panels = ex.ensure_panels(split_panel_bytes(a, p.prob.dtype, terms, p.tile));
+    hip_check(launch_contract_split(a, p.prob.dtype, terms, p.tile, panels, s), "split-BF16 contraction launch");
+  } else if (ksplit > 1) {
+    std::lock_guard<std::mutex> g(ex.panel_mu);
+    a.ksplit = ksplit;
+    a.kslabs = kslabs;
+    a.work = ex.ensure_panels((size_t)ksplit * (size_t)(a.M * a.N * a.H) * (size_t)es);
+    hip_check(launch_contract(a, p.prob.dtype, p.cfg.complex_method, p.tile, s), "contraction launch");
+    hip_check(launch_splitk_reduce(a, p.prob.dtype, s), "split-K reduction launch");
+  } else {
+    hip_check(launch_contract(a, p.prob.dtype, p.cfg.complex_method, p.tile, s), "contraction launch");
+  }
+}
+
 }  // namespace
 
 void validate_item(const Plan& p, const Item& it, bool beta_zero) {
@@ -251,29 +298,7 @@ void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in
   }
 
   ContractArgs a = p.args(ex.device, wa.dev, wb.dev, wc.dev, const_cast<void*>(wd.dev), alpha, beta, read_c);
-  if (!product) {
-    hip_check(launch_output_pass(a, p.prob.dtype, s), "output-pass launch");
-  } else if (p.strategy == 3) {
-    hip_check(launch_elementwise(a, p.prob.dtype, s), "elementwise launch");
-  } else if (p.cfg.precision == PREC_FP8) {
-    std::lock_guard<std::mutex> g(ex.panel_mu);
-    char* panels = ex.ensure_panels(fp8_panel_bytes(a, p.prob.dtype));
-    hip_check(launch_contract_fp8(a, p.prob.dtype, panels, s), "FP8 contraction launch");
-  } else if (p.cfg.precision == PREC_SPLIT_BF16_2 || p.cfg.precision == PREC_SPLIT_BF16_3) {
-    const int terms = p.cfg.precision == PREC_SPLIT_BF16_2 ? 2 : 3;
-    std::lock_guard<std::mutex> g(ex.panel_mu);
-    char* panels = ex.ensure_panels(split_panel_bytes(a, p.prob.dtype, terms, p.tile));
-    hip_check(launch_contract_split(a, p.prob.dtype, terms, p.tile, panels, s), "split-BF16 contraction launch");
-  } else if (p.ksplit > 1) {
-    std::lock_guard<std::mutex> g(ex.panel_mu);
-    a.ksplit = p.ksplit;
-    a.kslabs = p.kslabs;
-    a.work = ex.ensure_panels((size_t)p.ksplit * (size_t)(p.M * p.N * p.H) * (size_t)es);
-    hip_check(launch_contract(a, p.prob.dtype, p.cfg.complex_method, p.tile, s), "contraction launch");
-    hip_check(launch_splitk_reduce(a, p.prob.dtype, s), "split-K reduction launch");
-  } else {
-    hip_check(launch_contract(a, p.prob.dtype, p.cfg.complex_method, p.tile, s), "contraction launch");
-  }
+  launch_plan(p, ex, a, product, p.ksplit, p.kslabs, s);
 
   if (staged) {
     for (auto& st : stages)
@@ -286,6 +311,119 @@ void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in
   } else if (!ex.async.load()) {
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   }
+}
+
+bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double beta_in[2],
+                     const std::vector<Item>& items) {
+  const size_t n = items.size();
+  if (n < 2 || p.strategy == 2) return false;
+  double beta[2] = {beta_in[0], beta_in[1]};
+  for (const Item& it : items)
+    if (!it.c) beta[0] = beta[1] = 0.0;  // a null C needs beta 0 (validated), so C is never read
+  const bool read_c = !is_zero(beta);
+  const bool product = (p.strategy == 0 || p.strategy == 3) && !is_zero(alpha);
+  if (product && p.cfg.precision != PREC_STORAGE) return false;  // per-call operand packing
+  const int64_t es = elem_size(p.prob.dtype);
+  const int64_t H = p.H;
+  if ((int64_t)n > (INT64_MAX / 8) / std::max<int64_t>(H, 1)) return false;
+
+  // Every operand the launch touches must be device memory of this device, and
+  // every origin an element multiple away from the first item's.
+  DeviceGuard dg(ex.device);
+  auto device_origin = [&](const void* q, const void* q0, int64_t& delta) {
+    Where w = classify(q, ex.device);
+    if (w.host || w.dev != q) return false;
+    const intptr_t d = (intptr_t)q - (intptr_t)q0;
+    if (d % es != 0) return false;
+    delta = d / es;
+    return true;
+  };
+  std::vector<int64_t> da(n, 0), db(n, 0), dc(n, 0), dd(n, 0);
+  bool a16 = true, b16 = true;
+  for (size_t i = 0; i < n; i++) {
+    const Item& it = items[i];
+    if (product && !(device_origin(it.a, items[0].a, da[i]) && device_origin(it.b, items[0].b, db[i]))) return false;
+    if (read_c && !device_origin(it.c, items[0].c, dc[i])) return false;
+    if (!device_origin(it.d, items[0].d, dd[i])) return false;
+    a16 = a16 && (uintptr_t)it.a % 16 == 0;
+    b16 = b16 && (uintptr_t)it.b % 16 == 0;
+  }
+
+  // Running the items at once equals running them in order only if no item
+  // writes bytes that another item reads or writes (span hulls, conservative).
+  if (!p.prob.out_empty) {
+    std::vector<std::pair<Range, size_t>> outs;
+    outs.reserve(n);
+    for (size_t i = 0; i < n; i++) outs.push_back({byte_range(p, OP_D, items[i].d), i});
+    std::sort(outs.begin(), outs.end(), [](const auto& x, const auto& y) { return x.first.lo < y.first.lo; });
+    for (size_t i = 1; i < n; i++)
+      if (outs[i].first.lo < outs[i - 1].first.hi) return false;
+    auto hits_other_output = [&](const Range& r, size_t owner) {
+      if (!r.present) return false;
+      // The last output starting below r.hi; outputs are disjoint and sorted.
+      size_t lo = 0, hi = n;
+      while (lo < hi) {
+        const size_t mid = (lo + hi) / 2;
+        if (outs[mid].first.lo < r.hi) lo = mid + 1;
+        else hi = mid;
+      }
+      for (size_t k = lo; k-- > 0;) {
+        if (outs[k].first.hi <= r.lo) break;
+        if (outs[k].second != owner) return true;
+      }
+      return false;
+    };
+    for (size_t i = 0; i < n; i++) {
+      if (product && (hits_other_output(byte_range(p, OP_A, items[i].a), i) ||
+                      hits_other_output(byte_range(p, OP_B, items[i].b), i)))
+        return false;
+      if (read_c && hits_other_output(byte_range(p, OP_C, items[i].c), i)) return false;
+    }
+  }
+
+  // Batch offsets of item i, entry h: the item's origin delta plus the plan's
+  // own offset of entry h.
+  std::vector<int64_t> vec(4 * n * H);
+  int64_t* ha = vec.data();
+  int64_t* hb = ha + n * H;
+  int64_t* hc = hb + n * H;
+  int64_t* hd = hc + n * H;
+  for (size_t i = 0; i < n; i++)
+    for (int64_t h = 0; h < H; h++) {
+      const size_t e = i * H + h;
+      ha[e] = da[i] + (p.ha.empty() ? 0 : p.ha[h]);
+      hb[e] = db[i] + (p.hb.empty() ? 0 : p.hb[h]);
+      hc[e] = dc[i] + (p.hc.empty() ? 0 : p.hc[h]);
+      hd[e] = dd[i] + (p.hd.empty() ? 0 : p.hd[h]);
+    }
+
+  hipStream_t s = ex.get_stream();
+  std::lock_guard<std::mutex> g(ex.batch_mu);
+  int64_t* dev = (int64_t*)ex.ensure_batch(vec.size() * sizeof(int64_t));
+  hip_check(hipMemcpyAsync(dev, vec.data(), vec.size() * sizeof(int64_t), hipMemcpyHostToDevice, s),
+            "uploading batch offsets");
+  const Item& i0 = items[0];
+  ContractArgs a = p.args(ex.device, product ? i0.a : nullptr, product ? i0.b : nullptr, read_c ? i0.c : nullptr,
+                          i0.d, alpha, beta, read_c);
+  a.H = (int64_t)n * H;
+  a.ha = dev;
+  a.hb = dev + n * H;
+  a.hc = dev + 2 * n * H;
+  a.hd = dev + 3 * n * H;
+  a.h_affine = 0;
+  a.a_kvec = a.a_kvec && a16;
+  a.b_kvec = a.b_kvec && b16;
+  // The items multiply the tile count: K is split only while the whole batch
+  // would still leave the GPU underfilled.
+  int ksplit = p.ksplit;
+  if (ksplit > 1) {
+    const TileCfg* t = tile_cfg(p.tile);
+    const int64_t tiles = ((p.M + t->bm - 1) / t->bm) * ((p.N + t->bn - 1) / t->bn) * a.H;
+    if (tiles >= 128) ksplit = 1;
+  }
+  launch_plan(p, ex, a, product, ksplit, p.kslabs, s);
+  if (!ex.async.load()) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  return true;
 }
 
 }  // namespace tp

@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cstdint>
 #include <mutex>
+#include <vector>
 
 #include "plan.h"
 
@@ -30,10 +31,15 @@ struct Executor {
   std::mutex panel_mu;
   char* panels = nullptr;
   size_t panels_cap = 0;
+  // Batch-entry offset vectors of fused batched calls (reused in stream order).
+  std::mutex batch_mu;
+  char* batch_vecs = nullptr;
+  size_t batch_cap = 0;
 
   hipStream_t get_stream();
   char* ensure_scratch(size_t bytes);  // caller holds mu
   char* ensure_panels(size_t bytes);   // caller holds panel_mu
+  char* ensure_batch(size_t bytes);    // caller holds batch_mu
   ~Executor();
 };
 
@@ -54,6 +60,15 @@ void validate_item(const Plan& p, const Item& it, bool beta_zero);
 
 // Runs one validated item (plan/mod.rs:487-530 semantics).
 void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta[2], const Item& it);
+
+// Runs validated items of one plan as ONE launch when that gives the same
+// result as running them in order: every operand in device memory, storage
+// precision, element-aligned origins, and no item writing memory another item
+// reads or writes. The items become extra batch entries (their origins folded
+// into the batch offset vectors). Returns false, having done nothing, when the
+// items must run one by one.
+bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double beta[2],
+                     const std::vector<Item>& items);
 
 void hip_check(hipError_t e, const char* what);
 

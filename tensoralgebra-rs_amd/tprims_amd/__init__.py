@@ -375,6 +375,24 @@ class Plan:
         ex = executor.handle if executor is not None else 0
         check(lib().TAPP_execute_product(self.handle, ex, ctypes.byref(st), pa, a, b, pb, c, d))
 
+    def execute_batched_raw(self, alpha, a, b, beta, c, d, executor=None):
+        """TAPP_execute_batched_product (crates/tprims-capi/src/execute.rs:235):
+        the plan on len(d) items; a, b, d (and c, or None) are lists of element
+        addresses of index zero. Same alpha and beta for every item; items run
+        as one launch when that equals running them in order."""
+        dt = self.problem.dtype
+        n = len(d)
+        if len(a) != n or len(b) != n or (c is not None and len(c) != n):
+            raise ValueError("a, b, c and d need one address per item")
+        pa, _ka = _scalar(alpha, dt)
+        pb, _kb = _scalar(beta, dt)
+        arr = lambda xs: (ctypes.c_void_p * max(n, 1))(*[ctypes.c_void_p(x) for x in xs])
+        A, B, D = arr(a), arr(b), arr(d)
+        C = arr(c) if c is not None else None
+        st = _capi.intptr(0)
+        ex = executor.handle if executor is not None else 0
+        check(lib().TAPP_execute_batched_product(self.handle, ex, ctypes.byref(st), n, pa, A, B, pb, C, D))
+
     def execute_into(self, alpha, a, b, d, executor=None):
         """D = op_D(alpha * op_A(A) op_B(B)) (plan/mod.rs:271); C is not read."""
         self.execute_raw(alpha, _addr(a), _addr(b), 0.0, None, _addr(d), executor)
