@@ -9,6 +9,7 @@
 #include "plan.h"
 
 #include <algorithm>
+#include <cmath>
 #include <climits>
 #include <cstring>
 #include <memory>
@@ -138,16 +139,22 @@ int choose_tile(const Plan& p) {
   } else {
     cands = {{TILE_128x128, 1.3}, {TILE_128x64, 1.1}, {TILE_64x64, 1.0}, {TILE_32x32, 0.7}};
   }
+  // Estimated time of the MFMA work on the busiest CU: its tiles (the tile
+  // count spread over 256 CUs) times the tile area, over the SIMDs those
+  // tiles' waves can occupy (4 per CU) and the tile's measured efficiency. A
+  // big tile wins on big problems; on small ones it would leave CUs idle and
+  // compute its out-of-range rows, and a smaller tile finishes first.
   int best = 0;
-  double best_score = -1;
+  double best_cost = 0;
   for (auto [id, eff] : cands) {
     if (!tile_allowed(id, p.prob.dtype, prec, p.cfg.complex_method)) continue;
     const TileCfg* t = tile_cfg(id);
-    double wgs = (double)((p.M + t->bm - 1) / t->bm) * (double)((p.N + t->bn - 1) / t->bn) * (double)p.H;
-    double waves = wgs * (t->threads / 64);
-    double score = std::min(waves, 2048.0) * eff;
-    if (score > best_score + 1e-9) {
-      best_score = score;
+    const double tiles = (double)((p.M + t->bm - 1) / t->bm) * (double)((p.N + t->bn - 1) / t->bn) * (double)p.H;
+    const double per_cu = std::ceil(tiles / 256.0);
+    const double simds = std::min(4.0, per_cu * (t->threads / 64));
+    const double cost = per_cu * t->bm * t->bn / (simds * eff);
+    if (best == 0 || cost < best_cost * (1 - 1e-9)) {
+      best_cost = cost;
       best = id;
     }
   }
