@@ -116,20 +116,16 @@ struct DCfg {
   }
 };
 
+// One BM x BN output tile (tile row tm, column tn) of batch entry h over all
+// of K, staged through smem (C::LDS bytes), written back to D.
 template <class C>
-__global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
+__device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn, char* smem) {
   typedef typename C::Tv T;
   typedef typename C::W W;
   typedef Mma<T> MM;
   typedef typename MM::acc Acc;
   constexpr int BM = C::BM, BN = C::BN, BK = C::BK, NW = C::NW;
-  __shared__ __attribute__((aligned(16))) char smem[C::LDS];
   W* const lds = (W*)smem;
-
-  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
-  int64_t h;
-  int tm, tn;
-  dev::decode_tile(blockIdx.x, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = (wave % C::WAVES_M) * C::WM;
@@ -447,7 +443,62 @@ __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
 }
 
 template <class C>
+__global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
+  __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
+  int64_t h;
+  int tm, tn;
+  dev::decode_tile(blockIdx.x, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
+  dma_tile<C>(p, h, tm, tn, smem);
+}
+
+// The four gather orientations of one tile shape (A and B each K- or
+// M/N-major), chosen per tile from the item's arguments.
+template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH, bool PLR>
+struct DFamily {
+  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH, PLR> KK;
+  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH, PLR> KN;
+  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH, PLR> MK;
+  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR> MN;
+  static constexpr size_t m2(size_t x, size_t y) { return x > y ? x : y; }
+  static constexpr size_t LDS = m2(m2(KK::LDS, KN::LDS), m2(MK::LDS, MN::LDS));
+  static constexpr int NT = KK::NT;
+};
+
+// Persistent grouped kernel on the DMA tiles: workgroups stride over the
+// flattened tile space of every item; each tile finds its item by binary
+// search over tile_begin (every item has 32-bit gather offsets).
+template <class F>
+__global__ __launch_bounds__(F::NT) void k_grouped_dma(const GroupItem* __restrict__ items, int num_items,
+                                                       int64_t total_tiles) {
+  __shared__ __attribute__((aligned(16))) char smem[F::LDS];
+  for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    int lo = 0, hi = num_items - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].tile_begin <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const ContractArgs& p = items[lo].args;
+    const int64_t local = t - items[lo].tile_begin;
+    const int64_t per_h = (int64_t)p.tiles_m * p.tiles_n;
+    const int64_t h = local / per_h;
+    const int r = (int)(local - h * per_h);
+    const int tm = r % p.tiles_m, tn = r / p.tiles_m;
+    if (p.a_kmajor) {
+      if (p.b_kmajor) dma_tile<typename F::KK>(p, h, tm, tn, smem);
+      else dma_tile<typename F::KN>(p, h, tm, tn, smem);
+    } else {
+      if (p.b_kmajor) dma_tile<typename F::MK>(p, h, tm, tn, smem);
+      else dma_tile<typename F::MN>(p, h, tm, tn, smem);
+    }
+    __syncthreads();  // the next tile's DMA overwrites this tile's staging
+  }
+}
+
+template <class C>
 hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
+
   ContractArgs p = a;
   p.tiles_m = (int)((p.M + C::BM - 1) / C::BM);
   p.tiles_n = (int)((p.N + C::BN - 1) / C::BN);
@@ -491,10 +542,38 @@ hipError_t launch_complex(const ContractArgs& a, int tile_id, hipStream_t s) {
   }
 }
 
+template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH = CPLX_NATIVE, bool PLR = true>
+hipError_t launch_group_family(const GroupItem* items, int n, int64_t total, int grid, hipStream_t s) {
+  typedef DFamily<T, CPLX, BM, BN, BK, WM, WN, METH, PLR> F;
+  static_assert(F::LDS <= 160 * 1024, "LDS budget");
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_grouped_dma<F>, dim3((unsigned)grid), dim3(F::NT), 0, s, items, n, total);
+  return hipGetLastError();
+}
+
+// The grouped kernel's tile families: one wave layout for all four gather
+// orientations (the single-launch choices where they agree; f64 128x128 keeps
+// the 8-wave layout).
+template <typename T, bool CPLX, int METH>
+hipError_t group_by_tile(const GroupItem* items, int n, int64_t total, int tile_id, int grid, hipStream_t s) {
+  if constexpr (CPLX && sizeof(T) == 8) {
+    if (tile_id == TILE_128x128) {
+      if constexpr (METH == CPLX_NATIVE) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH>(items, n, total, grid, s);
+      else return launch_group_family<T, true, 128, 128, 16, 64, 32, METH, false>(items, n, total, grid, s);
+    }
+    return launch_group_family<T, true, 64, 64, 16, 32, 32, METH, METH != CPLX_3M>(items, n, total, grid, s);
+  } else {
+    if (tile_id == TILE_128x128)
+      return launch_group_family<T, CPLX, 128, 128, 16, 64, 32, METH, CPLX && METH != CPLX_NATIVE>(items, n, total, grid, s);
+    if (tile_id == TILE_128x64) return launch_group_family<T, CPLX, 128, 64, 16, 64, 32, METH>(items, n, total, grid, s);
+    return launch_group_family<T, CPLX, 64, 64, 16, 32, 32, METH>(items, n, total, grid, s);
+  }
+}
+
 }  // namespace
 
-// f32, c32 and c64 with the native schedule. On the complex benchmark the
-// c64 64x64 DMA kernel ran 2.41 ms against 2.98 ms for the register-staged
+// Every dtype and complex schedule with 32-bit gather offsets. On the complex
+// benchmark the c64 64x64 DMA kernel ran 2.41 ms against 2.98 ms for the register-staged
 // one on a GPU that holds its clock down under load (it moves no slab data
 // through VGPRs, and so draws less power), and 2.44-2.50 ms against 2.35 ms on
 // one that does not; the 128x128 tile brings it to 2.20 ms with 8 waves of
@@ -537,6 +616,23 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int t
   if (tile_id == TILE_128x128) return launch_majors<float, false, 128, 128, 16, 64, 32, CPLX_NATIVE, false>(a, s);  // prefetch: 122.7 -> 121.1
   if (tile_id == TILE_128x64) return launch_majors<float, false, 128, 64, 16, 64, 32>(a, s);
   return launch_majors<float, false, 64, 64, 16, 32, 32>(a, s);
+}
+
+hipError_t launch_grouped_dma(const GroupItem* items, int n, int64_t total, DType t, int method, int tile_id, int grid,
+                              hipStream_t s) {
+  switch (t) {
+    case DType::C64:
+      if (method == CPLX_3M) return group_by_tile<double, true, CPLX_3M>(items, n, total, tile_id, grid, s);
+      if (method == CPLX_1M) return group_by_tile<double, true, CPLX_1M>(items, n, total, tile_id, grid, s);
+      return group_by_tile<double, true, CPLX_NATIVE>(items, n, total, tile_id, grid, s);
+    case DType::C32:
+      if (method == CPLX_3M) return group_by_tile<float, true, CPLX_3M>(items, n, total, tile_id, grid, s);
+      if (method == CPLX_1M) return group_by_tile<float, true, CPLX_1M>(items, n, total, tile_id, grid, s);
+      return group_by_tile<float, true, CPLX_NATIVE>(items, n, total, tile_id, grid, s);
+    case DType::F64: return group_by_tile<double, false, CPLX_NATIVE>(items, n, total, tile_id, grid, s);
+    case DType::F32: return group_by_tile<float, false, CPLX_NATIVE>(items, n, total, tile_id, grid, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace tp

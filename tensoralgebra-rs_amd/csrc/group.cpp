@@ -36,6 +36,7 @@ struct Group {
   DType dtype = DType::F64;
   int method = 0;
   int tile = TILE_64x64;
+  bool all_off32 = true;  // every item's gathers fit 32-bit offsets: the DMA tiles apply
   int grid = 0;
   int64_t total_tiles = 0;
   int64_t flops = 0;
@@ -120,13 +121,17 @@ void build_work_list(Group& g) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, g.ex->device) == hipSuccess) cus = prop.multiProcessorCount;
   else (void)hipGetLastError();
-  const int per_cu = t->threads >= 256 ? 2 : 8;
+  const int per_cu = g.tile == TILE_128x128 ? 1 : (t->threads >= 256 ? 2 : 8);
   g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)cus * per_cu));
 }
 
 void launch_all(Group& g, hipStream_t s) {
   for (const ContractArgs& a : g.passes) hip_check(launch_output_pass(a, g.dtype, s), "output-pass launch");
-  if (g.n_items > 0)
+  if (g.n_items == 0) return;
+  if (g.all_off32 && dma_supported(true, g.dtype, g.method, g.tile))
+    hip_check(launch_grouped_dma(g.d_items, g.n_items, g.total_tiles, g.dtype, g.method, g.tile, g.grid, s),
+              "grouped launch");
+  else
     hip_check(launch_grouped(g.d_items, g.n_items, g.total_tiles, g.dtype, g.method, g.tile, g.grid, s),
               "grouped launch");
 }
@@ -156,6 +161,13 @@ void run_group(Group& g, Executor& ex) {
     launch_all(g, s);
   }
   if (!ex.async.load()) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+// Storage tiles a group can run: 128x64 is not offered for c64, and the
+// DMA-only 128x128 tile needs 32-bit gather offsets in every item.
+bool group_tile_ok(const Group& g, int id) {
+  if (id == TILE_128x128) return g.all_off32 && dma_supported(true, g.dtype, g.method, id);
+  return tile_cfg(id) != nullptr && id < 10 && !(id == TILE_128x64 && g.dtype == DType::C64);
 }
 
 Group* group_of(tprims_group h) {
@@ -215,17 +227,19 @@ TAPP_error tprims_group_create(tprims_group* out, TAPP_executor exec, int num_it
       if (p->strategy == 1 || (al[0] == 0 && al[1] == 0)) {
         g->passes.push_back(a);
       } else {
+        g->all_off32 = g->all_off32 && a.off32;
         g->args.push_back(a);
         g->own_tile.push_back(p->tile);
         g->flops += p->flops();
       }
     }
-    // Default tile: the one most items' plans chose.
+    // Default tile: the one most items' plans chose (the DMA-only 128x128 tile
+    // when every item can take it).
     int votes[16] = {0};
     for (Plan* p : g->plans)
-      if (p->tile > 0 && p->tile < 4) votes[p->tile]++;
-    g->tile = (int)(std::max_element(votes, votes + 4) - votes);
-    if (g->tile == 0 || (g->tile == TILE_128x64 && g->dtype == DType::C64)) g->tile = TILE_64x64;
+      if (p->tile > 0 && p->tile <= TILE_128x128) votes[p->tile]++;
+    g->tile = (int)(std::max_element(votes, votes + TILE_128x128 + 1) - votes);
+    if (g->tile == 0 || !group_tile_ok(*g, g->tile)) g->tile = TILE_64x64;
     build_work_list(*g);
     *out = (tprims_group)g.release();
   });
@@ -279,9 +293,7 @@ TAPP_error tprims_group_info(tprims_group group, int64_t* tiles, int64_t* flops,
 TAPP_error tprims_group_set_tile(tprims_group group, int32_t tile_id) {
   return ffi([&] {
     Group* g = group_of(group);
-    const TileCfg* t = tile_cfg(tile_id);
-    if (!t || tile_id >= 10 || (tile_id == TILE_128x64 && g->dtype == DType::C64))
-      fail(ERR_UNSUPPORTED, "tile id not available for grouped execution");
+    if (!group_tile_ok(*g, tile_id)) fail(ERR_UNSUPPORTED, "tile id not available for grouped execution");
     DeviceGuard dg(g->ex->device);
     g->tile = tile_id;
     build_work_list(*g);
@@ -303,8 +315,8 @@ TAPP_error tprims_group_autotune(tprims_group group, TAPP_executor exec, int rep
     hip_check(hipEventCreate(&e1), "hipEventCreate");
     int best = g->tile;
     float best_t = 1e30f;
-    for (int id : {TILE_64x64, TILE_32x32, TILE_128x64}) {
-      if (id == TILE_128x64 && g->dtype == DType::C64) continue;
+    for (int id : {TILE_128x128, TILE_64x64, TILE_32x32, TILE_128x64}) {
+      if (!group_tile_ok(*g, id)) continue;
       g->tile = id;
       build_work_list(*g);
       launch_all(*g, s);  // warm-up

@@ -68,5 +68,9 @@ hipError_t launch_zero_layout(void* base, DType t, int rank, const int64_t* dims
 // Persistent grouped kernel over `items` (device array) covering `total_tiles`.
 hipError_t launch_grouped(const GroupItem* items, int num_items, int64_t total_tiles, DType t, int method,
                           int tile_id, int grid, hipStream_t s);
+// The same on the LDS-DMA tiles (every item with 32-bit gather offsets and a
+// tile dma_supported accepts).
+hipError_t launch_grouped_dma(const GroupItem* items, int num_items, int64_t total_tiles, DType t, int method,
+                              int tile_id, int grid, hipStream_t s);
 
 }  // namespace tp

@@ -109,17 +109,46 @@ def test_group_autotune_keeps_results(gpu):
     plans, al, be, A, B, C, D, want = build(dtype, mps_items(dtype, 12, 9), gpu)
     g = tp.Group(plans, al, A, B, be, C, D)
     tile, ms = g.autotune(reps=2)
-    assert tile in (1, 2) and ms > 0
+    assert tile in (1, 2, 4) and ms > 0
     for d in D:
         d.zero_()
     g.execute()
     torch.cuda.synchronize()
     check_all(dtype, D, want, "autotuned")
-    for t in (1, 2):
+    for t in (1, 2, 4):
         g.set_tile(t)
         g.execute()
         torch.cuda.synchronize()
         check_all(dtype, D, want, "tile %d" % t)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [tp.DType.C64, tp.DType.F64, tp.DType.F32, tp.DType.C32], ids=lambda d: d.name)
+@pytest.mark.parametrize("method", [tp.ComplexMethod.NATIVE, tp.ComplexMethod.ONE_M, tp.ComplexMethod.THREE_M],
+                         ids=lambda m: m.name)
+def test_group_on_every_dma_tile(gpu, dtype, method):
+    """The grouped kernel on the LDS-DMA tiles (64x64, 128x64 where offered,
+    128x128) for every complex schedule, against the oracle."""
+    import torch
+
+    if not dtype.is_complex and method != tp.ComplexMethod.NATIVE:
+        pytest.skip("complex schedules apply to complex dtypes")
+    items = mps_items(dtype, 10, 21, dims=(16, 48, 80))
+    plans, al, be, A, B, C, D, want = build(dtype, items, gpu, beta=0.25)
+    if method != tp.ComplexMethod.NATIVE:
+        plans = [tp.Plan(p.problem, tp.PlanConfig(complex_method=method)) for p in plans]
+    g = tp.Group(plans, al, A, B, be, C, D)
+    for t in (1, 3, 4):
+        if t == 3 and dtype == tp.DType.C64:
+            continue
+        g.set_tile(t)
+        for d in D:
+            d.fill_(float("nan"))
+        g.execute()
+        torch.cuda.synchronize()
+        for k, (d, w) in enumerate(zip(D, want)):
+            err = oracle.max_rel_err(d.cpu().numpy().astype(np.complex128), w)
+            assert err <= tol(dtype, method), "tile %d item %d: %.3e" % (t, k, err)
 
 
 def test_group_rejects_mixed_dtypes_without_a_gpu_call():
