@@ -603,10 +603,12 @@ const TileCfg* tile_cfg(int id) {
 }
 
 hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s) {
-  if (a.ksplit <= 1 && dma_supported(a.off32, t, method, tile_id)) {
-    // The DMA kernel runs one tile per workgroup, within the launch limit.
+  if (dma_supported(a.off32, t, method, tile_id)) {
+    // The DMA kernel runs one tile (or one K range of one) per workgroup,
+    // within the launch limit.
     const TileCfg* c = tile_cfg(tile_id);
-    const int64_t nwg = ((a.M + c->bm - 1) / c->bm) * ((a.N + c->bn - 1) / c->bn) * a.H;
+    const int64_t nwg =
+        ((a.M + c->bm - 1) / c->bm) * ((a.N + c->bn - 1) / c->bn) * a.H * (a.ksplit > 1 ? a.ksplit : 1);
     if (nwg <= max_grid(c->threads)) return launch_contract_dma(a, t, method, tile_id, s);
   }
   // The DMA-only tile falls back to the register-staged 64x64 kernel.

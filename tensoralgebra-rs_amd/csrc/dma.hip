@@ -116,10 +116,12 @@ struct DCfg {
   }
 };
 
-// One BM x BN output tile (tile row tm, column tn) of batch entry h over all
-// of K, staged through smem (C::LDS bytes), written back to D.
+// One BM x BN output tile (tile row tm, column tn) of batch entry h, staged
+// through smem (C::LDS bytes): over all of K, written back to D; or, with
+// split K, over K range `split` (slabs [split * kslabs, + kslabs)), written
+// unscaled to the split-K workspace.
 template <class C>
-__device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn, char* smem) {
+__device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn, char* smem, int split = 0) {
   typedef typename C::Tv T;
   typedef typename C::W W;
   typedef Mma<T> MM;
@@ -355,12 +357,16 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   // read it), so the DMA of slab kt + 2 and the first fragments of slab
   // kt + 1 are issued under those MFMAs.
   constexpr int KQ = BK / KSTEP;
-  const int64_t nk = (K + BK - 1) / BK;
+  const int64_t nk_all = (K + BK - 1) / BK;
+  const int64_t kt0 = p.ksplit > 1 ? (int64_t)split * p.kslabs : 0;
+  const int64_t nk_rest = nk_all - kt0;
+  const int64_t nk = p.ksplit > 1 ? (nk_rest < p.kslabs ? (nk_rest > 0 ? nk_rest : 0) : p.kslabs) : nk_all;
+  const int64_t kb0 = kt0 * BK;  // first k of the range
   uint32_t a_ko[C::JA], b_ko[C::JB];
-  {
+  if (nk > 0) {
     uint32_t a1[C::JA], b1[C::JB];
-    load_koff(0, a_ko, b_ko);
-    if (nk > 1) load_koff(BK, a1, b1);
+    load_koff(kb0, a_ko, b_ko);
+    if (nk > 1) load_koff(kb0 + BK, a1, b1);
     issue(0, a_ko, b_ko);
     if (nk > 1) issue(1, a1, b1);
   }
@@ -371,7 +377,7 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
     for (int64_t kt = 0; kt < nk; kt++) {
       const bool more = kt + 2 < nk;
       const int st = (int)(kt & 1);
-      if (more) load_koff((kt + 2) * BK, a_ko, b_ko);
+      if (more) load_koff(kb0 + (kt + 2) * BK, a_ko, b_ko);
 #pragma unroll
       for (int q = 0; q < KQ; q++) {
         load_frag(st, KSTEP * q, f[0]);
@@ -382,11 +388,11 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
       if (more) issue(st, a_ko, b_ko);
     }
   } else {
-  load_frag(0, 0, f[0]);
+  if (nk > 0) load_frag(0, 0, f[0]);
   for (int64_t kt = 0; kt < nk; kt++) {
     const bool more = kt + 2 < nk;
     const int st = (int)(kt & 1);
-    if (more) load_koff((kt + 2) * BK, a_ko, b_ko);  // lands during the MFMAs
+    if (more) load_koff(kb0 + (kt + 2) * BK, a_ko, b_ko);  // lands during the MFMAs
 #pragma unroll
     for (int q = 0; q < KQ; q++) {
       if (q + 1 < KQ) {
@@ -438,7 +444,10 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
           }
     }
     __syncthreads();
-    dev::write_back<T, T, C::CPLX, ER, BN, C::NT, C::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
+    if (p.ksplit > 1)
+      dev::write_partial<T, C::CPLX, ER, BN, C::NT, C::LDD>(p, split, h, m0 + c * ER, n0, Dre, Dim);
+    else
+      dev::write_back<T, T, C::CPLX, ER, BN, C::NT, C::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
   }
 }
 
@@ -446,10 +455,12 @@ template <class C>
 __global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
   __shared__ __attribute__((aligned(16))) char smem[C::LDS];
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
+  // With split K the splits of a tile are ksplit consecutive ids.
+  const int S = p.ksplit > 1 ? p.ksplit : 1;
   int64_t h;
   int tm, tn;
-  dev::decode_tile(blockIdx.x, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
-  dma_tile<C>(p, h, tm, tn, smem);
+  dev::decode_tile((int64_t)blockIdx.x / S, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
+  dma_tile<C>(p, h, tm, tn, smem, (int)(blockIdx.x % S));
 }
 
 // The four gather orientations of one tile shape (A and B each K- or
@@ -502,7 +513,7 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   ContractArgs p = a;
   p.tiles_m = (int)((p.M + C::BM - 1) / C::BM);
   p.tiles_n = (int)((p.N + C::BN - 1) / C::BN);
-  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
+  const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H * (p.ksplit > 1 ? p.ksplit : 1);
   if (nwg == 0) return hipSuccess;
   if (nwg > max_grid(C::NT)) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL(k_contract_dma<C>, dim3((unsigned)nwg), dim3(C::NT), 0, s, p);

@@ -272,9 +272,8 @@ void configure(Plan* plan, const PlanConfig& cfg) {
   // Split K when fewer than 128 tiles would run (half of the 256 CUs) and
   // every split keeps at least 6 slabs: about 512 workgroups (two per CU), at
   // most 16 splits (f64 576^3: 81 tiles x 6 splits measured fastest of 1..18).
-  // Split tiles run on the register-staged kernel, so a planner-chosen 128x128
-  // tile (DMA only) gives way to 64x64 here. cfg.split_k = 1 keeps one K
-  // range, n > 1 asks for n splits (at most one per slab).
+  // cfg.split_k = 1 keeps one K range, n > 1 asks for n splits (at most one
+  // per slab).
   p.ksplit = 1;
   p.kslabs = 0;
   auto tiles_of = [&](int id) {
@@ -282,10 +281,7 @@ void configure(Plan* plan, const PlanConfig& cfg) {
     return ((p.M + tc->bm - 1) / tc->bm) * ((p.N + tc->bn - 1) / tc->bn) * p.H;
   };
   if (cfg.split_k < 0 || cfg.split_k > 64) fail(ERR_INVALID_ARGUMENT, "split_k must be in [0, 64]");
-  if (p.strategy == 0 && cfg.precision == PREC_STORAGE && cfg.tile == 0 && p.tile == TILE_128x128 &&
-      (cfg.split_k > 1 || (cfg.split_k == 0 && tiles_of(TILE_128x128) < 128 && (p.K + 15) / 16 >= 16)))
-    p.tile = TILE_64x64;
-  if (p.strategy == 0 && cfg.precision == PREC_STORAGE && p.tile != TILE_128x128 && cfg.split_k != 1) {
+  if (p.strategy == 0 && cfg.precision == PREC_STORAGE && cfg.split_k != 1) {
     const TileCfg* tc = tile_cfg(p.tile);
     const int64_t tiles = tiles_of(p.tile);
     const int64_t nk = (p.K + tc->bk - 1) / tc->bk;
@@ -450,7 +446,7 @@ std::string Plan::kernel_name() const {
   const bool fallback = tile == TILE_128x128 && !dma_supported(off32, prob.dtype, cfg.complex_method, tile);
   const TileCfg* t = tile_cfg(fallback ? TILE_64x64 : tile);
   if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(cfg.precision == PREC_FP8 ? 128 : t->bk);
-  if (cfg.precision == PREC_STORAGE && !fallback && ksplit <= 1 && dma_supported(off32, prob.dtype, cfg.complex_method, tile))
+  if (cfg.precision == PREC_STORAGE && !fallback && dma_supported(off32, prob.dtype, cfg.complex_method, tile))
     s += ".dma";
   if (ksplit > 1) s += ".splitk" + std::to_string(ksplit);
   return s;
