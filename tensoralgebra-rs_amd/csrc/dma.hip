@@ -54,11 +54,13 @@ __device__ inline void dma_load(__amdgpu_buffer_rsrc_t r, void* dst, uint32_t of
 
 // PLR_: read each k step's fragments while the previous step's MFMAs run
 // (two fragment sets in registers); off where the accumulators leave no room.
+// MINW_: waves per SIMD the register allocation must allow (1: no bound).
 template <typename T, bool CPLX_, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_, int METH_,
-          bool PLR_ = true>
+          bool PLR_ = true, int MINW_ = 1>
 struct DCfg {
   typedef T Tv;
   static constexpr bool CPLX = CPLX_, AKM = AKM_, BKM = BKM_, PLR = PLR_;
+  static constexpr int MINW = MINW_;
   static constexpr int METH = CPLX_ ? METH_ : CPLX_NATIVE;
   static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
   static constexpr int WAVES_M = BM / WM;
@@ -120,7 +122,7 @@ struct DCfg {
 // through smem (C::LDS bytes): over all of K, written back to D; or, with
 // split K, over K range `split` (slabs [split * kslabs, + kslabs)), written
 // unscaled to the split-K workspace.
-template <class C>
+template <class C, bool SPLIT = false>
 __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn, char* smem, int split = 0) {
   typedef typename C::Tv T;
   typedef typename C::W W;
@@ -358,9 +360,9 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   // kt + 1 are issued under those MFMAs.
   constexpr int KQ = BK / KSTEP;
   const int64_t nk_all = (K + BK - 1) / BK;
-  const int64_t kt0 = p.ksplit > 1 ? (int64_t)split * p.kslabs : 0;
+  const int64_t kt0 = SPLIT ? (int64_t)split * p.kslabs : 0;
   const int64_t nk_rest = nk_all - kt0;
-  const int64_t nk = p.ksplit > 1 ? (nk_rest < p.kslabs ? (nk_rest > 0 ? nk_rest : 0) : p.kslabs) : nk_all;
+  const int64_t nk = SPLIT ? (nk_rest < p.kslabs ? (nk_rest > 0 ? nk_rest : 0) : p.kslabs) : nk_all;
   const int64_t kb0 = kt0 * BK;  // first k of the range
   uint32_t a_ko[C::JA], b_ko[C::JB];
   if (nk > 0) {
@@ -444,23 +446,24 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
           }
     }
     __syncthreads();
-    if (p.ksplit > 1)
+    if constexpr (SPLIT)
       dev::write_partial<T, C::CPLX, ER, BN, C::NT, C::LDD>(p, split, h, m0 + c * ER, n0, Dre, Dim);
     else
       dev::write_back<T, T, C::CPLX, ER, BN, C::NT, C::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
   }
 }
 
-template <class C>
-__global__ __launch_bounds__(C::NT) void k_contract_dma(ContractArgs p) {
+// SPLIT: K split into p.ksplit ranges, the splits of a tile being ksplit
+// consecutive workgroup ids.
+template <class C, bool SPLIT>
+__global__ __launch_bounds__(C::NT, C::MINW) void k_contract_dma(ContractArgs p) {
   __shared__ __attribute__((aligned(16))) char smem[C::LDS];
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
-  // With split K the splits of a tile are ksplit consecutive ids.
-  const int S = p.ksplit > 1 ? p.ksplit : 1;
+  const int S = SPLIT ? p.ksplit : 1;
   int64_t h;
   int tm, tn;
   dev::decode_tile((int64_t)blockIdx.x / S, nwg, p.tiles_m, p.tiles_n, h, tm, tn);
-  dma_tile<C>(p, h, tm, tn, smem, (int)(blockIdx.x % S));
+  dma_tile<C, SPLIT>(p, h, tm, tn, smem, (int)(blockIdx.x % S));
 }
 
 // The four gather orientations of one tile shape (A and B each K- or
@@ -516,7 +519,8 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H * (p.ksplit > 1 ? p.ksplit : 1);
   if (nwg == 0) return hipSuccess;
   if (nwg > max_grid(C::NT)) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(k_contract_dma<C>, dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
+  if (p.ksplit > 1) hipLaunchKernelGGL((k_contract_dma<C, true>), dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
+  else hipLaunchKernelGGL((k_contract_dma<C, false>), dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
   return hipGetLastError();
 }
 
@@ -618,8 +622,9 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int t
         if (a.b_kmajor) return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, true, CPLX_NATIVE>>(a, s);
         return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, false, CPLX_NATIVE>>(a, s);
       }
-      if (a.b_kmajor) return launch_cfg<DCfg<double, false, 128, 128, 16, 64, 32, false, true, CPLX_NATIVE, false>>(a, s);
-      return launch_cfg<DCfg<double, false, 128, 128, 16, 64, 32, false, false, CPLX_NATIVE, false>>(a, s);
+      // Two workgroups per CU (LDS allows it) need at most 128 VGPRs.
+      if (a.b_kmajor) return launch_cfg<DCfg<double, false, 128, 128, 16, 64, 32, false, true, CPLX_NATIVE, false, 4>>(a, s);
+      return launch_cfg<DCfg<double, false, 128, 128, 16, 64, 32, false, false, CPLX_NATIVE, false, 4>>(a, s);
     }
     if (tile_id == TILE_128x64) return launch_majors<double, false, 128, 64, 16, 64, 32>(a, s);
     return launch_majors<double, false, 64, 64, 16, 32, 32>(a, s);
