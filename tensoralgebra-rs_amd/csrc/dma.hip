@@ -359,10 +359,13 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   // read it), so the DMA of slab kt + 2 and the first fragments of slab
   // kt + 1 are issued under those MFMAs.
   constexpr int KQ = BK / KSTEP;
+  // p.kslabs counts slabs of 16 k (the planner's unit); a split's range is a
+  // whole number of this kernel's slabs (the launcher checks).
   const int64_t nk_all = (K + BK - 1) / BK;
-  const int64_t kt0 = SPLIT ? (int64_t)split * p.kslabs : 0;
+  const int64_t ks = SPLIT ? p.kslabs * 16 / BK : 0;
+  const int64_t kt0 = SPLIT ? (int64_t)split * ks : 0;
   const int64_t nk_rest = nk_all - kt0;
-  const int64_t nk = SPLIT ? (nk_rest < p.kslabs ? (nk_rest > 0 ? nk_rest : 0) : p.kslabs) : nk_all;
+  const int64_t nk = SPLIT ? (nk_rest < ks ? (nk_rest > 0 ? nk_rest : 0) : ks) : nk_all;
   const int64_t kb0 = kt0 * BK;  // first k of the range
   uint32_t a_ko[C::JA], b_ko[C::JB];
   if (nk > 0) {
@@ -512,7 +515,7 @@ __global__ __launch_bounds__(F::NT) void k_grouped_dma(const GroupItem* __restri
 
 template <class C>
 hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
-
+  if (a.ksplit > 1 && (a.kslabs * 16) % C::BK != 0) return hipErrorInvalidValue;  // split range not whole slabs
   ContractArgs p = a;
   p.tiles_m = (int)((p.M + C::BM - 1) / C::BM);
   p.tiles_n = (int)((p.N + C::BN - 1) / C::BN);

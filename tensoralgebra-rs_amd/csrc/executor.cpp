@@ -85,11 +85,31 @@ char* Executor::ensure_batch(size_t bytes) {
   return batch_vecs;
 }
 
+char* Executor::ensure_batch_host(size_t bytes) {
+  if (batch_ev_pending) {
+    hip_check(hipEventSynchronize(batch_ev), "hipEventSynchronize");
+    batch_ev_pending = false;
+  }
+  if (!batch_ev) hip_check(hipEventCreateWithFlags(&batch_ev, hipEventDisableTiming), "hipEventCreate");
+  if (bytes <= batch_host_cap) return batch_host;
+  if (batch_host) {
+    hip_check(hipHostFree(batch_host), "hipHostFree");
+    batch_host = nullptr;
+    batch_host_cap = 0;
+  }
+  size_t cap = bytes < ((size_t)1 << 16) ? ((size_t)1 << 16) : bytes + bytes / 8;
+  hip_check(hipHostMalloc((void**)&batch_host, cap, hipHostMallocDefault), "hipHostMalloc(batch offsets)");
+  batch_host_cap = cap;
+  return batch_host;
+}
+
 Executor::~Executor() {
   if (stream_ready && stream) (void)hipStreamSynchronize(stream);
   if (scratch) (void)hipFree(scratch);
   if (panels) (void)hipFree(panels);
   if (batch_vecs) (void)hipFree(batch_vecs);
+  if (batch_host) (void)hipHostFree(batch_host);
+  if (batch_ev) (void)hipEventDestroy(batch_ev);
   if (owns_stream && stream_ready && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -382,9 +402,12 @@ bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double 
   }
 
   // Batch offsets of item i, entry h: the item's origin delta plus the plan's
-  // own offset of entry h.
-  std::vector<int64_t> vec(4 * n * H);
-  int64_t* ha = vec.data();
+  // own offset of entry h, written to pinned memory and uploaded in stream
+  // order.
+  hipStream_t s = ex.get_stream();
+  std::lock_guard<std::mutex> g(ex.batch_mu);
+  const size_t bytes = 4 * n * H * sizeof(int64_t);
+  int64_t* ha = (int64_t*)ex.ensure_batch_host(bytes);
   int64_t* hb = ha + n * H;
   int64_t* hc = hb + n * H;
   int64_t* hd = hc + n * H;
@@ -397,11 +420,10 @@ bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double 
       hd[e] = dd[i] + (p.hd.empty() ? 0 : p.hd[h]);
     }
 
-  hipStream_t s = ex.get_stream();
-  std::lock_guard<std::mutex> g(ex.batch_mu);
-  int64_t* dev = (int64_t*)ex.ensure_batch(vec.size() * sizeof(int64_t));
-  hip_check(hipMemcpyAsync(dev, vec.data(), vec.size() * sizeof(int64_t), hipMemcpyHostToDevice, s),
-            "uploading batch offsets");
+  int64_t* dev = (int64_t*)ex.ensure_batch(bytes);
+  hip_check(hipMemcpyAsync(dev, ha, bytes, hipMemcpyHostToDevice, s), "uploading batch offsets");
+  hip_check(hipEventRecord(ex.batch_ev, s), "hipEventRecord");
+  ex.batch_ev_pending = true;
   const Item& i0 = items[0];
   ContractArgs a = p.args(ex.device, product ? i0.a : nullptr, product ? i0.b : nullptr, read_c ? i0.c : nullptr,
                           i0.d, alpha, beta, read_c);

@@ -31,15 +31,22 @@ struct Executor {
   std::mutex panel_mu;
   char* panels = nullptr;
   size_t panels_cap = 0;
-  // Batch-entry offset vectors of fused batched calls (reused in stream order).
+  // Batch-entry offset vectors of fused batched calls: a device buffer reused
+  // in stream order, filled from a pinned host buffer whose previous upload
+  // (batch_ev) has completed before it is rewritten.
   std::mutex batch_mu;
   char* batch_vecs = nullptr;
   size_t batch_cap = 0;
+  char* batch_host = nullptr;
+  size_t batch_host_cap = 0;
+  hipEvent_t batch_ev = nullptr;
+  bool batch_ev_pending = false;
 
   hipStream_t get_stream();
   char* ensure_scratch(size_t bytes);  // caller holds mu
   char* ensure_panels(size_t bytes);   // caller holds panel_mu
-  char* ensure_batch(size_t bytes);    // caller holds batch_mu
+  char* ensure_batch(size_t bytes);       // caller holds batch_mu
+  char* ensure_batch_host(size_t bytes);  // caller holds batch_mu; waits for the previous upload
   ~Executor();
 };
 
