@@ -327,3 +327,30 @@ def test_split_k_choice_and_override():
     assert make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=3)).report()["kernel"].endswith(".splitk3")
     with pytest.raises(tp.TprimsError):
         make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=-1))
+
+
+def test_tile_choice_follows_the_busiest_cu_estimate():
+    """Large products take the 128x128 DMA tile; small ones a tile that does
+    not leave one CU computing an oversized tile (estimated MFMA time of the
+    busiest CU: tiles per CU x tile area over the SIMDs their waves occupy)."""
+    import tprims_amd as tp
+    from harness import T, make_plan
+
+    def gemm(dt, m, n, k, h=1):
+        return make_plan(dt, T([m, k, h], [1, m, m * k], [0, 2, 3]), T([k, n, h], [1, k, k * n], [2, 1, 3]),
+                         T([m, n, h], [1, m, m * n], [0, 1, 3]), config=tp.PlanConfig(split_k=1)).report()
+
+    for dt in (tp.DType.F32, tp.DType.F64, tp.DType.C32, tp.DType.C64):
+        big = gemm(dt, 4096, 4096, 256)
+        assert big["tile"] == 4 and big["kernel"].endswith("128x128x16.dma"), big["kernel"]
+    # 512 x 512 x 16 batches fill the GPU with 128x128 tiles (the headline shape)
+    assert gemm(tp.DType.C64, 512, 512, 64, 16)["tile"] == 4
+    # one 64x64 output: a 64x64 tile on 4 SIMDs, not a 128x128 one
+    small = gemm(tp.DType.C64, 64, 64, 64)
+    assert small["tile"] == 1 and small["kernel"].endswith("64x64x16.dma"), small["kernel"]
+    # 128 tiles of 128x128 would leave half the CUs idle: 64x64 spreads over all
+    assert gemm(tp.DType.C64, 256, 256, 1024, 32)["tile"] == 1
+    # an explicit tile is kept
+    p = make_plan(tp.DType.F64, T([64, 64], [1, 64], [0, 2]), T([64, 64], [1, 64], [2, 1]),
+                  T([64, 64], [1, 64], [0, 1]), config=tp.PlanConfig(tile=4)).report()
+    assert p["tile"] == 4

@@ -43,7 +43,7 @@ def _items(torch, dtype, a, b, d, n, seed, gap=3):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
-@pytest.mark.parametrize("shape", ["gemm", "batched_strided", "split_k", "elementwise"])
+@pytest.mark.parametrize("shape", ["gemm", "batched_strided", "split_k", "elementwise", "negative_strides"])
 def test_batched_items_match_the_oracle(gpu, dtype, shape):
     import torch
 
@@ -55,6 +55,11 @@ def test_batched_items_match_the_oracle(gpu, dtype, shape):
         b = T([33, 18, 3], [18, 1, 33 * 18], [2, 1, 3])
         d = T([20, 18, 3], [1, 20, 360], [0, 1, 3])
         n = 5
+    elif shape == "negative_strides":  # 64-bit gathers (register-staged kernel)
+        a = T([20, 30], [-1, -20], [0, 2], conj=dtype.is_complex)
+        b = T([30, 24], [24, 1], [2, 1])
+        d = T([20, 24], [-24, 1], [0, 1])
+        n = 6
     elif shape == "split_k":
         a, b, d = T([40, 900], [1, 40], [0, 2]), T([900, 24], [1, 900], [2, 1]), T([40, 24], [1, 40], [0, 1])
         n = 3
