@@ -540,11 +540,12 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
 // One complex method's tiles (c64: 64x64, and 128x128 of 16 waves for native,
 // 8 waves of 64x32 for 1m and 3m, whose accumulators do not fit the 128
 // registers a lane of a 16-wave workgroup has, and then leave no room for a
-// second fragment set; c32: 64x64, 128x64 and 128x128 of 8 waves). Fragment
-// prefetch measured on the complex benchmark contraction (same GPU, same
-// run): c64 native 128x128 62.5 -> 64.3 TFLOP/s, c32 1m 128x128 83 -> 86;
-// c32 native 128x128 103 -> 95 and c64 3m 64x64 63 -> 54 (one wave per SIMD),
-// so those read each step as they go.
+// second fragment set; c32: 64x64, 128x64 of 8 waves and 128x128 of 16 waves
+// of 32x32). Measured on the complex benchmark contraction (same GPU, same
+// run): fragment prefetch c64 native 128x128 62.5 -> 64.3 TFLOP/s, but c64 3m
+// 64x64 63 -> 54 (one wave per SIMD); c32 128x128 with 16 waves instead of 8
+// of 64x32: native 101 -> 109.5 and 3m 105 -> 119 reading each step as it
+// goes, 1m 89 -> 97 with prefetch.
 template <typename T, int METH>
 hipError_t launch_complex(const ContractArgs& a, int tile_id, hipStream_t s) {
   if constexpr (sizeof(T) == 8) {
@@ -554,7 +555,7 @@ hipError_t launch_complex(const ContractArgs& a, int tile_id, hipStream_t s) {
     }
     return launch_majors<T, true, 64, 64, 16, 32, 32, METH, METH != CPLX_3M>(a, s);
   } else {
-    if (tile_id == TILE_128x128) return launch_majors<T, true, 128, 128, 16, 64, 32, METH, METH != CPLX_NATIVE>(a, s);
+    if (tile_id == TILE_128x128) return launch_majors<T, true, 128, 128, 16, 32, 32, METH, METH == CPLX_1M>(a, s);
     if (tile_id == TILE_128x64) return launch_majors<T, true, 128, 64, 16, 64, 32, METH>(a, s);
     return launch_majors<T, true, 64, 64, 16, 32, 32, METH>(a, s);
   }
@@ -581,8 +582,10 @@ hipError_t group_by_tile(const GroupItem* items, int n, int64_t total, int tile_
     }
     return launch_group_family<T, true, 64, 64, 16, 32, 32, METH, METH != CPLX_3M>(items, n, total, grid, s);
   } else {
-    if (tile_id == TILE_128x128)
-      return launch_group_family<T, CPLX, 128, 128, 16, 64, 32, METH, CPLX && METH != CPLX_NATIVE>(items, n, total, grid, s);
+    if (tile_id == TILE_128x128) {
+      if constexpr (CPLX) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH, METH == CPLX_1M>(items, n, total, grid, s);
+      else return launch_group_family<T, false, 128, 128, 16, 64, 32, METH, false>(items, n, total, grid, s);
+    }
     if (tile_id == TILE_128x64) return launch_group_family<T, CPLX, 128, 64, 16, 64, 32, METH>(items, n, total, grid, s);
     return launch_group_family<T, CPLX, 64, 64, 16, 32, 32, METH>(items, n, total, grid, s);
   }

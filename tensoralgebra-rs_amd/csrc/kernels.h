@@ -20,7 +20,7 @@ enum TileId : int {
   TILE_64x64 = 1,  // 4 waves of 32x32
   TILE_32x32 = 2,  // 1 wave of 32x32
   TILE_128x64 = 3, // 4 waves of 64x32
-  TILE_128x128 = 4, // LDS-DMA kernel only (8 waves of 64x32; c64 native 16 of 32x32)
+  TILE_128x128 = 4, // LDS-DMA kernel only (16 waves of 32x32, or 8 of 64x32: by dtype, schedule, majors)
 };
 // Tile ids of the split-BF16 kernels (16x16x32 bf16 MFMA family).
 enum SplitTileId : int {
