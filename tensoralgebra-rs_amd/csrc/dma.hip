@@ -537,10 +537,10 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
   return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR>>(a, s);
 }
 
-// One complex method's tiles (c64: 64x64, and 128x128 of 16 waves for native,
-// 8 waves of 64x32 for 1m and 3m, whose accumulators do not fit the 128
-// registers a lane of a 16-wave workgroup has, and then leave no room for a
-// second fragment set; c32: 64x64, 128x64 of 8 waves and 128x128 of 16 waves
+// One complex method's tiles (c64: 64x64, and 128x128 of 16 waves of 32x32
+// for native and 3m (3m without fragment prefetch: 80.8 -> 86.1 TFLOP/s
+// against 8 waves of 64x32), 8 waves of 64x32 for 1m (58 against 51 with 16
+// waves); c32: 64x64, 128x64 of 8 waves and 128x128 of 16 waves
 // of 32x32). Measured on the complex benchmark contraction (same GPU, same
 // run): fragment prefetch c64 native 128x128 62.5 -> 64.3 TFLOP/s, but c64 3m
 // 64x64 63 -> 54 (one wave per SIMD); c32 128x128 with 16 waves instead of 8
@@ -551,6 +551,7 @@ hipError_t launch_complex(const ContractArgs& a, int tile_id, hipStream_t s) {
   if constexpr (sizeof(T) == 8) {
     if (tile_id == TILE_128x128) {
       if constexpr (METH == CPLX_NATIVE) return launch_majors<T, true, 128, 128, 16, 32, 32, METH>(a, s);
+      else if constexpr (METH == CPLX_3M) return launch_majors<T, true, 128, 128, 16, 32, 32, METH, false>(a, s);
       else return launch_majors<T, true, 128, 128, 16, 64, 32, METH, false>(a, s);
     }
     return launch_majors<T, true, 64, 64, 16, 32, 32, METH, METH != CPLX_3M>(a, s);
@@ -578,6 +579,7 @@ hipError_t group_by_tile(const GroupItem* items, int n, int64_t total, int tile_
   if constexpr (CPLX && sizeof(T) == 8) {
     if (tile_id == TILE_128x128) {
       if constexpr (METH == CPLX_NATIVE) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH>(items, n, total, grid, s);
+      else if constexpr (METH == CPLX_3M) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH, false>(items, n, total, grid, s);
       else return launch_group_family<T, true, 128, 128, 16, 64, 32, METH, false>(items, n, total, grid, s);
     }
     return launch_group_family<T, true, 64, 64, 16, 32, 32, METH, METH != CPLX_3M>(items, n, total, grid, s);
