@@ -537,7 +537,9 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
   return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR>>(a, s);
 }
 
-// One complex method's tiles (c64: 64x64, and 128x128 of 16 waves of 32x32
+// One complex method's tiles (c64: 64x64 of 8 waves of 32x16 (headline shape
+// on 64x64 tiles 54.5 -> 58 TFLOP/s, 512-item MPS group 44.5 -> 46.6 against
+// 4 waves of 32x32), and 128x128 of 16 waves of 32x32
 // for native and 3m (3m without fragment prefetch: 80.8 -> 86.1 TFLOP/s
 // against 8 waves of 64x32), 8 waves of 64x32 for 1m (58 against 51 with 16
 // waves); c32: 64x64, 128x64 of 8 waves and 128x128 of 16 waves
@@ -554,7 +556,7 @@ hipError_t launch_complex(const ContractArgs& a, int tile_id, hipStream_t s) {
       else if constexpr (METH == CPLX_3M) return launch_majors<T, true, 128, 128, 16, 32, 32, METH, false>(a, s);
       else return launch_majors<T, true, 128, 128, 16, 64, 32, METH, false>(a, s);
     }
-    return launch_majors<T, true, 64, 64, 16, 32, 32, METH, METH != CPLX_3M>(a, s);
+    return launch_majors<T, true, 64, 64, 16, 32, 16, METH, METH != CPLX_3M>(a, s);
   } else {
     if (tile_id == TILE_128x128) return launch_majors<T, true, 128, 128, 16, 32, 32, METH, METH == CPLX_1M>(a, s);
     if (tile_id == TILE_128x64) return launch_majors<T, true, 128, 64, 16, 64, 32, METH>(a, s);
@@ -582,7 +584,7 @@ hipError_t group_by_tile(const GroupItem* items, int n, int64_t total, int tile_
       else if constexpr (METH == CPLX_3M) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH, false>(items, n, total, grid, s);
       else return launch_group_family<T, true, 128, 128, 16, 64, 32, METH, false>(items, n, total, grid, s);
     }
-    return launch_group_family<T, true, 64, 64, 16, 32, 32, METH, METH != CPLX_3M>(items, n, total, grid, s);
+    return launch_group_family<T, true, 64, 64, 16, 32, 16, METH, METH != CPLX_3M>(items, n, total, grid, s);
   } else {
     if (tile_id == TILE_128x128) {
       if constexpr (CPLX) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH, METH == CPLX_1M>(items, n, total, grid, s);
