@@ -9,7 +9,11 @@
 //   crates/tprims-contract/src/driver/tile.rs:142     K accumulation
 //
 // One workgroup computes one BM x BN tile of one batch entry h over the whole
-// K range (K is never split, so the result does not depend on the grid). Each
+// K range, or with split K over one of ksplit ranges (partial tiles to the
+// workspace, reduced in split order afterwards), so the result does not
+// depend on the grid. These kernels serve operands whose gathers need 64-bit
+// offsets, the 32x32 tile and the grouped launches whose items do; the
+// LDS-DMA kernels of dma.hip serve the rest. Each
 // K step gathers a BM x BK slab of A and a BK x BN slab of B straight from HBM
 // through the scatter vectors into LDS planes, folding conjugation and the 3m
 // sum plane into the store. The lane order of a gather follows the operand's

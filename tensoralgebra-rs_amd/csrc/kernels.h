@@ -39,8 +39,9 @@ inline int64_t max_grid(int threads) { return (int64_t)((0xffffffffull / (unsign
 // The contraction kernel of a storage dtype, complex method and tile.
 // Returns hipSuccess or the launch error.
 hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s);
-// The LDS-DMA variant of the storage kernels (f32, c64 native; 32-bit
-// gather offsets), which launch_contract picks whenever it applies.
+// The LDS-DMA variant of the storage kernels (every dtype and complex method,
+// tiles 64x64, 128x64 (not c64) and 128x128; 32-bit gather offsets), which
+// launch_contract picks whenever it applies, split K included.
 bool dma_supported(bool off32, DType t, int method, int tile_id);
 hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s);
 // Split-BF16 for f32/c32 storage (nterms 2 or 3): A and B are split once into
