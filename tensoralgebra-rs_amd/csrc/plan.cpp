@@ -269,9 +269,10 @@ void configure(Plan* plan, const PlanConfig& cfg) {
   // (the K order is the same for both tiles: BK 16).
   if (p.tile == TILE_128x128 && !p.off32 && cfg.tile == 0) p.tile = TILE_64x64;
 
-  // Split K when fewer than 128 tiles would run (half of the 256 CUs) and
-  // every split keeps at least 6 slabs: about 512 workgroups (two per CU), at
-  // most 16 splits (f64 576^3: 81 tiles x 6 splits measured fastest of 1..18).
+  // Split K when fewer than 128 tiles would run (half of the 256 CUs): as many
+  // splits as keep every tile's splits in one round over the 256 CUs, each
+  // split at least 6 slabs, at most 16 (f64 576^3, 81 tiles: 3 splits 25.4 us
+  // per call, 4 splits 30.5 (a second round on 68 CUs), 6 splits 27.6).
   // cfg.split_k = 1 keeps one K range, n > 1 asks for n splits (at most one
   // per slab).
   p.ksplit = 1;
@@ -287,7 +288,7 @@ void configure(Plan* plan, const PlanConfig& cfg) {
     const int64_t nk = (p.K + tc->bk - 1) / tc->bk;
     int64_t s = 1;
     if (cfg.split_k > 1) s = std::min<int64_t>(cfg.split_k, nk);
-    else if (tiles < 128 && nk >= 16) s = std::min<int64_t>(std::min<int64_t>((512 + tiles - 1) / tiles, nk / 6), 16);
+    else if (tiles < 128 && nk >= 16) s = std::min<int64_t>(std::min<int64_t>(256 / tiles, nk / 6), 16);
     if (s >= 2) {
       p.ksplit = (int)s;
       p.kslabs = (nk + s - 1) / s;

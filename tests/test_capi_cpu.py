@@ -322,9 +322,9 @@ def test_split_k_choice_and_override():
     e = 24
     sp = [1, e, e * e, e ** 3]
     args = (T([e] * 4, sp, [0, 4, 1, 5]), T([e] * 4, sp, [5, 2, 4, 3]), T([e] * 4, sp, [0, 1, 2, 3]))
-    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.64x64x16.dma.splitk6"
+    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.64x64x16.dma.splitk3"
     assert ".splitk" not in make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=1)).report()["kernel"]
-    assert make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=3)).report()["kernel"].endswith(".splitk3")
+    assert make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=5)).report()["kernel"].endswith(".splitk5")
     with pytest.raises(tp.TprimsError):
         make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=-1))
 
