@@ -541,8 +541,8 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
 // on 64x64 tiles 54.5 -> 58 TFLOP/s, 512-item MPS group 44.5 -> 46.6 against
 // 4 waves of 32x32), and 128x128 of 16 waves of 32x32
 // for native and 3m (3m without fragment prefetch: 80.8 -> 86.1 TFLOP/s
-// against 8 waves of 64x32), 8 waves of 64x32 for 1m (58 against 51 with 16
-// waves); c32: 64x64, 128x64 of 8 waves and 128x128 of 16 waves
+// against 8 waves of 64x32), 8 waves of 32x64 with fragment prefetch for 1m
+// (60.7 against 58 for 64x32 and 51 for 16 waves of 32x32); c32: 64x64, 128x64 of 8 waves and 128x128 of 16 waves
 // of 32x32). Measured on the complex benchmark contraction (same GPU, same
 // run): fragment prefetch c64 native 128x128 62.5 -> 64.3 TFLOP/s, but c64 3m
 // 64x64 63 -> 54 (one wave per SIMD); c32 128x128 with 16 waves instead of 8
@@ -554,7 +554,7 @@ hipError_t launch_complex(const ContractArgs& a, int tile_id, hipStream_t s) {
     if (tile_id == TILE_128x128) {
       if constexpr (METH == CPLX_NATIVE) return launch_majors<T, true, 128, 128, 16, 32, 32, METH>(a, s);
       else if constexpr (METH == CPLX_3M) return launch_majors<T, true, 128, 128, 16, 32, 32, METH, false>(a, s);
-      else return launch_majors<T, true, 128, 128, 16, 64, 32, METH, false>(a, s);
+      else return launch_majors<T, true, 128, 128, 16, 32, 64, METH, true>(a, s);
     }
     return launch_majors<T, true, 64, 64, 16, 32, 16, METH, METH != CPLX_3M>(a, s);
   } else {
@@ -582,7 +582,7 @@ hipError_t group_by_tile(const GroupItem* items, int n, int64_t total, int tile_
     if (tile_id == TILE_128x128) {
       if constexpr (METH == CPLX_NATIVE) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH>(items, n, total, grid, s);
       else if constexpr (METH == CPLX_3M) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH, false>(items, n, total, grid, s);
-      else return launch_group_family<T, true, 128, 128, 16, 64, 32, METH, false>(items, n, total, grid, s);
+      else return launch_group_family<T, true, 128, 128, 16, 32, 64, METH, true>(items, n, total, grid, s);
     }
     return launch_group_family<T, true, 64, 64, 16, 32, 16, METH, METH != CPLX_3M>(items, n, total, grid, s);
   } else {
