@@ -68,7 +68,8 @@ struct PackArgs {
   int64_t RB;  // row blocks (of 16) per (h, K block) in the panel, padded
   int64_t KB;  // K blocks of 32
   int32_t conj;
-  int32_t vec;  // ContractArgs::a_kvec / b_kvec of this operand
+  int32_t vec;     // ContractArgs::a_kvec / b_kvec of this operand
+  int32_t kmajor;  // ContractArgs::a_kmajor / b_kmajor: contiguous along K
   __bf16* out;
 };
 
@@ -146,6 +147,68 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
     }
 #pragma unroll
     for (int x = 0; x < TERMS; x++) *(v8bf*)(out + (c * TERMS + x) * FRAG) = t[x];
+  }
+}
+
+// The same panels for operands without 16-byte K vectors: a workgroup first
+// reads its 64-row x 32-k block with lanes along the operand's contiguous
+// axis (K when KMAJ, the rows otherwise), so that every load instruction is
+// one run of nearby addresses where the layout has one, and stages it in LDS
+// (row pitch 33 floats); each lane then takes its row and 8 k from there.
+template <bool CPLX, int TERMS, bool KMAJ>
+__global__ __launch_bounds__(256) void k_pack_lds(PackArgs p) {
+  typedef Store<float, CPLX> St;
+  typedef typename St::S S;
+  constexpr int COMPS = CPLX ? 2 : 1;
+  constexpr int PER = COMPS * TERMS;
+  constexpr int LDW = KBLK + 1;
+  __shared__ float tre[64 * LDW];
+  __shared__ float tim[CPLX ? 64 * LDW : 1];
+  const int64_t rgroups = p.RB / 4;
+  int64_t bid = blockIdx.x;
+  const int64_t rg = bid % rgroups;
+  bid /= rgroups;
+  const int64_t kb = bid % p.KB;
+  const int64_t h = bid / p.KB;
+  const int64_t r0 = rg * 64, kg0 = kb * KBLK;
+  // Row 0 and k 0 have offset 0: out-of-range lanes read X[0] and store zero.
+  const S* __restrict__ X = (const S*)p.X + p.hv[h];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    int rl, kl;
+    if constexpr (KMAJ) {
+      kl = threadIdx.x & 31;
+      rl = (threadIdx.x >> 5) + 8 * i;
+    } else {
+      rl = threadIdx.x & 63;
+      kl = (threadIdx.x >> 6) + 4 * i;
+    }
+    const int64_t r = r0 + rl, k = kg0 + kl;
+    const bool ok = r < p.R && k < p.K;
+    const S v = X[ok ? p.rv[r] + p.kv[k] : 0];
+    tre[rl * LDW + kl] = ok ? St::re(v) : 0.0f;
+    if constexpr (CPLX) tim[rl * LDW + kl] = ok ? St::im(v) : 0.0f;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rl = w * 16 + (lane & 15), kq = 8 * (lane >> 4);
+  const int64_t rb = rg * 4 + w;
+  __bf16* out = p.out + ((h * p.KB + kb) * p.RB + rb) * PER * FRAG + lane * 8;
+#pragma unroll
+  for (int c = 0; c < COMPS; c++) {
+    const float* t = c == 0 ? tre : tim;
+    v8bf o[TERMS];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      float v = t[rl * LDW + kq + j];
+      if (c == 1 && p.conj) v = -v;
+      __bf16 s[TERMS];
+      split<TERMS>(v, s);
+#pragma unroll
+      for (int x = 0; x < TERMS; x++) o[x][j] = s[x];
+    }
+#pragma unroll
+    for (int x = 0; x < TERMS; x++) *(v8bf*)(out + (c * TERMS + x) * FRAG) = o[x];
   }
 }
 
@@ -365,12 +428,17 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
   const int64_t RBa = pad(a.M, sh.bm), RBb = pad(a.N, sh.bn);
   __bf16* Ap = (__bf16*)panels;
   __bf16* Bp = Ap + a.H * KB * RBa * PER * FRAG;
-  PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kvec, Ap};
-  PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, Bp};
+  PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kvec, a.a_kmajor, Ap};
+  PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, a.b_kmajor, Bp};
   for (const PackArgs* q : {&pa, &pb}) {
     const int64_t grid = q->H * q->KB * (q->RB / 4);
     if (grid > max_grid(256)) return hipErrorInvalidConfiguration;
     if (q->vec) hipLaunchKernelGGL((k_pack<CPLX, TERMS, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
+    // K-major without 16-byte vectors: lanes along K through LDS (c32
+    // benchmark contraction, 3 terms: 110 -> 129 TFLOP/s). M-major operands
+    // keep the direct gather, whose 16-row runs measured faster there (f32
+    // transform step: 98 against 93).
+    else if (q->kmajor) hipLaunchKernelGGL((k_pack_lds<CPLX, TERMS, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     else hipLaunchKernelGGL((k_pack<CPLX, TERMS, false>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
