@@ -53,7 +53,8 @@ struct F8PackArgs {
   int64_t RB;  // row blocks (of 16) per (h, K block), padded to whole tiles and to 4
   int64_t KB;  // K blocks of 128
   int32_t conj;
-  int32_t vec;  // ContractArgs::a_kvec / b_kvec of this operand
+  int32_t vec;     // ContractArgs::a_kvec / b_kvec of this operand
+  int32_t kmajor;  // ContractArgs::a_kmajor / b_kmajor: contiguous along K
   uint8_t* out;
   int32_t* scales;  // [h][kb][RB * 16 rows][comp]
 };
@@ -143,6 +144,83 @@ __global__ __launch_bounds__(256) void k_pack_fp8(F8PackArgs p) {
     v4u* dst = (v4u*)(out + c * F8_FRAG);
     dst[0] = v4u{(unsigned)w[0], (unsigned)w[1], (unsigned)w[2], (unsigned)w[3]};
     dst[1] = v4u{(unsigned)w[4], (unsigned)w[5], (unsigned)w[6], (unsigned)w[7]};
+    if (lane < 16) p.scales[((h * p.KB + kb) * p.RB * 16 + r) * COMPS + c] = 127 + e;
+  }
+}
+
+// The same panels for K-major operands without 16-byte K vectors: the
+// workgroup walks its 64-row x 128-k block in four 32-k chunks, each read
+// with lanes along K (one run of nearby addresses per load instruction) into
+// LDS (row pitch 33 floats); the lanes whose 32 k lie in the chunk take their
+// row from there.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_pack_fp8_lds(F8PackArgs p) {
+  typedef Store<float, CPLX> St;
+  typedef typename St::S S;
+  constexpr int COMPS = CPLX ? 2 : 1;
+  constexpr int LDW = 33;
+  __shared__ float tre[64 * LDW];
+  __shared__ float tim[CPLX ? 64 * LDW : 1];
+  const int lane = threadIdx.x & 63;
+  const int64_t rgroups = p.RB / 4;
+  int64_t bid = blockIdx.x;
+  const int64_t rg = bid % rgroups;
+  bid /= rgroups;
+  const int64_t kb = bid % p.KB;
+  const int64_t h = bid / p.KB;
+  const int w = threadIdx.x >> 6, lk = lane >> 4;
+  const int64_t rb = rg * 4 + w;
+  const int64_t r = rb * 16 + (lane & 15);
+  const int rl = w * 16 + (lane & 15);
+  // Row 0 and k 0 have offset 0: out-of-range lanes read X[0] and store zero.
+  const S* __restrict__ X = (const S*)p.X + p.hv[h];
+  float re[32], im[32];
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++) {
+    if (ch > 0) __syncthreads();  // the previous chunk has been read
+    const int kl = threadIdx.x & 31;
+    const int64_t k = kb * F8_KBLK + 32 * ch + kl;
+    const bool kok = k < p.K;
+    const int64_t ko = kok ? p.kv[k] : 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int rr = (threadIdx.x >> 5) + 8 * i;
+      const int64_t rg_r = rg * 64 + rr;
+      const bool ok = kok && rg_r < p.R;
+      const S v = X[ok ? p.rv[rg_r] + ko : 0];
+      tre[rr * LDW + kl] = ok ? St::re(v) : 0.0f;
+      if constexpr (CPLX) tim[rr * LDW + kl] = ok ? St::im(v) : 0.0f;
+    }
+    __syncthreads();
+    if (lk == ch) {
+#pragma unroll
+      for (int j = 0; j < 32; j++) {
+        re[j] = tre[rl * LDW + j];
+        im[j] = CPLX ? tim[rl * LDW + j] : 0.0f;
+      }
+    }
+  }
+  uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rb) * COMPS * F8_FRAG + lane * 32;
+#pragma unroll
+  for (int c = 0; c < COMPS; c++) {
+    const bool neg = c == 1 && p.conj;
+    float amax = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 32; j++) amax = fmaxf(amax, fabsf(c == 0 ? re[j] : im[j]));
+    amax = fmaxf(amax, __shfl_xor(amax, 16));
+    amax = fmaxf(amax, __shfl_xor(amax, 32));
+    const int e = panel_exponent(amax);
+    const float inv = ldexpf(neg ? -1.0f : 1.0f, -e);
+    int wd[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+      const float* v = (c == 0 ? re : im) + 4 * d;
+      int x = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, 0, false);
+      wd[d] = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, x, true);
+    }
+    v4u* dst = (v4u*)(out + c * F8_FRAG);
+    dst[0] = v4u{(unsigned)wd[0], (unsigned)wd[1], (unsigned)wd[2], (unsigned)wd[3]};
+    dst[1] = v4u{(unsigned)wd[4], (unsigned)wd[5], (unsigned)wd[6], (unsigned)wd[7]};
     if (lane < 16) p.scales[((h * p.KB + kb) * p.RB * 16 + r) * COMPS + c] = 127 + e;
   }
 }
@@ -321,12 +399,13 @@ hipError_t run(const ContractArgs& a, void* panels, hipStream_t s) {
   uint8_t* Bp = Ap + a.H * KB * RBa * COMPS * F8_FRAG;
   int32_t* As = (int32_t*)(Bp + a.H * KB * RBb * COMPS * F8_FRAG);
   int32_t* Bs = As + a.H * KB * RBa * 16 * COMPS;
-  F8PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kvec, Ap, As};
-  F8PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, Bp, Bs};
+  F8PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kvec, a.a_kmajor, Ap, As};
+  F8PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, a.b_kmajor, Bp, Bs};
   for (const F8PackArgs* q : {&pa, &pb}) {
     const int64_t grid = q->H * q->KB * (q->RB / 4);
     if (grid > max_grid(256)) return hipErrorInvalidConfiguration;
     if (q->vec) hipLaunchKernelGGL((k_pack_fp8<CPLX, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
+    else if (q->kmajor) hipLaunchKernelGGL((k_pack_fp8_lds<CPLX>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     else hipLaunchKernelGGL((k_pack_fp8<CPLX, false>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
