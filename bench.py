@@ -19,6 +19,7 @@ contractions measured the same way:
               one persistent launch (autotuned tile, hipGraph) vs one launch each
   tapp_batched  512 c64 64^3 GEMMs of one plan through TAPP_execute_batched_product
               (one launch) vs one call per item
+  elementwise f64 Hadamard D[i] = A[i] B[i], 2^27 elements (HBM-bound)
   vjp         D[i,k,b] = sum_{j,r} A[i,i,j,r,b] B[j,k,b] forward + dA, dB (f32, c64)
 with relative errors against FP64 evaluations of sampled outputs, and the CPU
 baseline (the C++ port of the reference's packed driver on the host cores).
@@ -294,6 +295,26 @@ def config_tapp_batched(torch, tp, np, ex, stream, steps, n=512, e=64):
             "speedup": round(t_each / t_fused, 2), "rel_err_item": err, "kernel": plan.report()["kernel"]}
 
 
+def config_elementwise(torch, tp, np, ex, stream, steps, n=1 << 27):
+    """The elementwise (all-batch) strategy on a Hadamard product of 2^27 f64
+    (D[i] = A[i] B[i]): HBM-bound, 24 bytes per element."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(10)
+    A = torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+    B = torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+    D = torch.empty(n, dtype=torch.float64, device=dev)
+    sp = lambda: tp.OperandSpec(tp.LayoutSpec([n], [1]))
+    plan = tp.Plan(tp.Problem.from_labels(tp.DType.F64, sp(), sp(), tp.CSpec.separate(sp()), sp(),
+                                          tp.Labels([0], [0], [0], [0])))
+    run = lambda: plan.execute_raw(1.0, A.data_ptr(), B.data_ptr(), 0.0, None, D.data_ptr(), ex)
+    ms = timed(torch, run, stream, max(3, steps // 2), 2)
+    ok = bool(torch.equal(D, A * B))
+    gbs = 24 * n / ms / 1e6
+    return {"workload": "f64 Hadamard D[i]=A[i]B[i], 2^27 elements", "ms": round(ms, 4),
+            "hbm_gbs": round(gbs, 1), "frac_hbm_peak": round(gbs / 8000.0, 4), "exact": ok,
+            "kernel": plan.report()["kernel"]}
+
+
 def config_vjp(torch, tp, np, ex, stream, steps):
     dev = torch.device("cuda:0")
     out = {"workload": "D[i,k,b]=sum_{j,r} A[i,i,j,r,b] B[j,k,b], i=k=256, j=128, r=8, b=32: forward + dA, dB"}
@@ -455,6 +476,7 @@ def main():
                          ("gemm4096", lambda: config_precision(torch, tp, np, ex, stream, args.steps, "gemm")),
                          ("grouped", lambda: config_grouped(torch, tp, np, ex, stream, args.steps)),
                          ("tapp_batched", lambda: config_tapp_batched(torch, tp, np, ex, stream, args.steps)),
+                         ("elementwise", lambda: config_elementwise(torch, tp, np, ex, stream, args.steps)),
                          ("vjp", lambda: config_vjp(torch, tp, np, ex, stream, args.steps))):
             try:
                 cfgs[name] = fn()

@@ -93,7 +93,7 @@ def test_every_tile(gpu, tile, dtype):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
-@pytest.mark.parametrize("layout", ["affine", "two_batch_axes", "hadamard"])
+@pytest.mark.parametrize("layout", ["affine", "two_batch_axes", "hadamard", "hadamard_dense"])
 def test_elementwise_strategy(gpu, dtype, layout):
     """The all-batch strategy (tiny M x N and K, many batch entries) against the
     oracle: conj(A), beta * op_C(C), one folded batch axis (affine offsets) or
@@ -102,6 +102,9 @@ def test_elementwise_strategy(gpu, dtype, layout):
     if layout == "hadamard":
         n = 5000
         a, b, d = T([n], [1], [0], conj=dtype.is_complex), T([n], [2], [0]), T([n], [1], [0])
+    elif layout == "hadamard_dense":  # unit strides: 16-byte vectors and a ragged end
+        n = 5003
+        a, b, d = T([n], [1], [0], conj=dtype.is_complex), T([n], [1], [0]), T([n], [1], [0])
     elif layout == "affine":
         H = 3000
         a = T([2, 3, H], [1, 2, 6], [0, 2, 3], conj=dtype.is_complex)
