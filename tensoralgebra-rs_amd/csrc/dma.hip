@@ -471,22 +471,23 @@ __global__ __launch_bounds__(C::NT, C::MINW) void k_contract_dma(ContractArgs p)
 
 // The four gather orientations of one tile shape (A and B each K- or
 // M/N-major), chosen per tile from the item's arguments.
-template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH, bool PLR>
+template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH, bool PLR, int MW>
 struct DFamily {
-  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH, PLR> KK;
-  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH, PLR> KN;
-  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH, PLR> MK;
-  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR> MN;
+  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH, PLR, MW> KK;
+  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH, PLR, MW> KN;
+  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH, PLR, MW> MK;
+  typedef DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR, MW> MN;
   static constexpr size_t m2(size_t x, size_t y) { return x > y ? x : y; }
   static constexpr size_t LDS = m2(m2(KK::LDS, KN::LDS), m2(MK::LDS, MN::LDS));
   static constexpr int NT = KK::NT;
+  static constexpr int MINW = KK::MINW;
 };
 
 // Persistent grouped kernel on the DMA tiles: workgroups stride over the
 // flattened tile space of every item; each tile finds its item by binary
 // search over tile_begin (every item has 32-bit gather offsets).
 template <class F>
-__global__ __launch_bounds__(F::NT) void k_grouped_dma(const GroupItem* __restrict__ items, int num_items,
+__global__ __launch_bounds__(F::NT, F::MINW) void k_grouped_dma(const GroupItem* __restrict__ items, int num_items,
                                                        int64_t total_tiles) {
   __shared__ __attribute__((aligned(16))) char smem[F::LDS];
   for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
@@ -527,83 +528,111 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH = CPLX_NATIVE, bool PLR = true>
+template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH, bool PLR, int MINW>
 hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
   if (a.a_kmajor) {
-    if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH, PLR>>(a, s);
-    return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH, PLR>>(a, s);
+    if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH, PLR, MINW>>(a, s);
+    return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH, PLR, MINW>>(a, s);
   }
-  if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH, PLR>>(a, s);
-  return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR>>(a, s);
+  if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH, PLR, MINW>>(a, s);
+  return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR, MINW>>(a, s);
 }
 
-// One complex method's tiles (c64: 64x64 of 8 waves of 32x16 (headline shape
-// on 64x64 tiles 54.5 -> 58 TFLOP/s, 512-item MPS group 44.5 -> 46.6 against
-// 4 waves of 32x32), and 128x128 of 16 waves of 32x32
-// for native and 3m (3m without fragment prefetch: 80.8 -> 86.1 TFLOP/s
-// against 8 waves of 64x32), 8 waves of 32x64 with fragment prefetch for 1m
-// (60.7 against 58 for 64x32 and 51 for 16 waves of 32x32); c32: 64x64, 128x64 of 8 waves and 128x128 of 16 waves
-// of 32x32). Measured on the complex benchmark contraction (same GPU, same
-// run): fragment prefetch c64 native 128x128 62.5 -> 64.3 TFLOP/s, but c64 3m
-// 64x64 63 -> 54 (one wave per SIMD); c32 128x128 with 16 waves instead of 8
-// of 64x32: native 101 -> 109.5 and 3m 105 -> 119 reading each step as it
-// goes, 1m 89 -> 97 with prefetch.
-template <typename T, int METH>
-hipError_t launch_complex(const ContractArgs& a, int tile_id, hipStream_t s) {
-  if constexpr (sizeof(T) == 8) {
-    if (tile_id == TILE_128x128) {
-      if constexpr (METH == CPLX_NATIVE) return launch_majors<T, true, 128, 128, 16, 32, 32, METH>(a, s);
-      else if constexpr (METH == CPLX_3M) return launch_majors<T, true, 128, 128, 16, 32, 32, METH, false>(a, s);
-      else return launch_majors<T, true, 128, 128, 16, 32, 64, METH, true>(a, s);
-    }
-    return launch_majors<T, true, 64, 64, 16, 32, 16, METH, METH != CPLX_3M>(a, s);
-  } else {
-    if (tile_id == TILE_128x128) return launch_majors<T, true, 128, 128, 16, 32, 32, METH, METH == CPLX_1M>(a, s);
-    if (tile_id == TILE_128x64) return launch_majors<T, true, 128, 64, 16, 64, 32, METH>(a, s);
-    return launch_majors<T, true, 64, 64, 16, 32, 32, METH>(a, s);
-  }
-}
-
-template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH = CPLX_NATIVE, bool PLR = true>
-hipError_t launch_group_family(const GroupItem* items, int n, int64_t total, int grid, hipStream_t s) {
-  typedef DFamily<T, CPLX, BM, BN, BK, WM, WN, METH, PLR> F;
-  static_assert(F::LDS <= 160 * 1024, "LDS budget");
-  if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_grouped_dma<F>, dim3((unsigned)grid), dim3(F::NT), 0, s, items, n, total);
-  return hipGetLastError();
-}
-
-// The grouped kernel's tile families: one wave layout for all four gather
-// orientations (the single-launch choices where they agree; f64 128x128 keeps
-// the 8-wave layout).
-template <typename T, bool CPLX, int METH>
-hipError_t group_by_tile(const GroupItem* items, int n, int64_t total, int tile_id, int grid, hipStream_t s) {
+// The wave layout of every DMA tile (BM x BN over waves of WM x WN, fragment
+// prefetch PLR, register bound MINW waves per SIMD) per dtype and complex
+// schedule, shared by single and grouped launches: act.go<...>() launches it.
+// Each choice was measured against its alternatives on the benchmark shapes
+// (same GPU, same run):
+//  - c64 128x128: 16 waves of 32x32, prefetch for native (62.5 -> 64.3
+//    TFLOP/s against 8 waves of 64x32), none for 3m (80.8 -> 86.1); 1m 8
+//    waves of 32x64 with prefetch (60.7 against 58 for 64x32, 51 for 16
+//    waves). c64 64x64: 8 waves of 32x16 (54.5 -> 58 against 4 of 32x32;
+//    3m without prefetch: 63 against 54).
+//  - c32 128x128: 16 waves of 32x32 (native 101 -> 109.5 and 3m 105 -> 119
+//    without prefetch, 1m 89 -> 97 with it).
+//  - f64 128x128 (grouped, or M-major A): 8 waves of 64x32 without prefetch,
+//    at most 128 VGPRs so two workgroups share a CU; f64 64x64: 8 waves of
+//    32x16 (f64 24^6 31.2 -> 28.4 us against 4 of 32x32).
+//  - f32 128x128: 8 waves of 64x32 without prefetch (122.7 against 121.1).
+template <typename T, bool CPLX, int METH, class Act>
+hipError_t by_tile(int tile_id, const Act& act) {
   if constexpr (CPLX && sizeof(T) == 8) {
     if (tile_id == TILE_128x128) {
-      if constexpr (METH == CPLX_NATIVE) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH>(items, n, total, grid, s);
-      else if constexpr (METH == CPLX_3M) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH, false>(items, n, total, grid, s);
-      else return launch_group_family<T, true, 128, 128, 16, 32, 64, METH, true>(items, n, total, grid, s);
+      if constexpr (METH == CPLX_NATIVE) return act.template go<128, 128, 32, 32, true, 1>();
+      else if constexpr (METH == CPLX_3M) return act.template go<128, 128, 32, 32, false, 1>();
+      else return act.template go<128, 128, 32, 64, true, 1>();
     }
-    return launch_group_family<T, true, 64, 64, 16, 32, 16, METH, METH != CPLX_3M>(items, n, total, grid, s);
+    return act.template go<64, 64, 32, 16, METH != CPLX_3M, 1>();
+  } else if constexpr (CPLX) {
+    if (tile_id == TILE_128x128) return act.template go<128, 128, 32, 32, METH == CPLX_1M, 1>();
+    if (tile_id == TILE_128x64) return act.template go<128, 64, 64, 32, true, 1>();
+    return act.template go<64, 64, 32, 32, true, 1>();
+  } else if constexpr (sizeof(T) == 8) {
+    if (tile_id == TILE_128x128) return act.template go<128, 128, 64, 32, false, 4>();
+    if (tile_id == TILE_128x64) return act.template go<128, 64, 64, 32, true, 1>();
+    return act.template go<64, 64, 32, 16, true, 1>();
   } else {
-    if (tile_id == TILE_128x128) {
-      if constexpr (CPLX) return launch_group_family<T, true, 128, 128, 16, 32, 32, METH, METH == CPLX_1M>(items, n, total, grid, s);
-      else return launch_group_family<T, false, 128, 128, 16, 64, 32, METH, false>(items, n, total, grid, s);
-    }
-    if (tile_id == TILE_128x64) return launch_group_family<T, CPLX, 128, 64, 16, 64, 32, METH>(items, n, total, grid, s);
-    return launch_group_family<T, CPLX, 64, 64, 16, 32, 32, METH>(items, n, total, grid, s);
+    if (tile_id == TILE_128x128) return act.template go<128, 128, 64, 32, false, 1>();
+    if (tile_id == TILE_128x64) return act.template go<128, 64, 64, 32, true, 1>();
+    return act.template go<64, 64, 32, 32, true, 1>();
   }
+}
+
+template <typename T, bool CPLX, int METH>
+struct Single {
+  const ContractArgs& a;
+  hipStream_t s;
+  template <int BM, int BN, int WM, int WN, bool PLR, int MINW>
+  hipError_t go() const {
+    return launch_majors<T, CPLX, BM, BN, 16, WM, WN, METH, PLR, MINW>(a, s);
+  }
+};
+
+template <typename T, bool CPLX, int METH>
+struct Grouped {
+  const GroupItem* items;
+  int n;
+  int64_t total;
+  int grid;
+  hipStream_t s;
+  template <int BM, int BN, int WM, int WN, bool PLR, int MINW>
+  hipError_t go() const {
+    typedef DFamily<T, CPLX, BM, BN, 16, WM, WN, METH, PLR, MINW> F;
+    static_assert(F::LDS <= 160 * 1024, "LDS budget");
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grouped_dma<F>, dim3((unsigned)grid), dim3(F::NT), 0, s, items, n, total);
+    return hipGetLastError();
+  }
+};
+
+template <typename T, bool CPLX>
+hipError_t single_by_method(const ContractArgs& a, int method, int tile_id, hipStream_t s) {
+  if constexpr (CPLX) {
+    if (method == CPLX_3M) return by_tile<T, true, CPLX_3M>(tile_id, Single<T, true, CPLX_3M>{a, s});
+    if (method == CPLX_1M) return by_tile<T, true, CPLX_1M>(tile_id, Single<T, true, CPLX_1M>{a, s});
+  }
+  return by_tile<T, CPLX, CPLX_NATIVE>(tile_id, Single<T, CPLX, CPLX_NATIVE>{a, s});
+}
+
+template <typename T, bool CPLX>
+hipError_t grouped_by_method(const GroupItem* items, int n, int64_t total, int method, int tile_id, int grid,
+                             hipStream_t s) {
+  if constexpr (CPLX) {
+    if (method == CPLX_3M)
+      return by_tile<T, true, CPLX_3M>(tile_id, Grouped<T, true, CPLX_3M>{items, n, total, grid, s});
+    if (method == CPLX_1M)
+      return by_tile<T, true, CPLX_1M>(tile_id, Grouped<T, true, CPLX_1M>{items, n, total, grid, s});
+  }
+  return by_tile<T, CPLX, CPLX_NATIVE>(tile_id, Grouped<T, CPLX, CPLX_NATIVE>{items, n, total, grid, s});
 }
 
 }  // namespace
 
 // Every dtype and complex schedule with 32-bit gather offsets. On the complex
-// benchmark the c64 64x64 DMA kernel ran 2.41 ms against 2.98 ms for the register-staged
-// one on a GPU that holds its clock down under load (it moves no slab data
-// through VGPRs, and so draws less power), and 2.44-2.50 ms against 2.35 ms on
-// one that does not; the 128x128 tile brings it to 2.20 ms with 8 waves of
-// 64x32 and 2.13 ms with 16 waves of 32x32 (c64's choice; f32 and c32 keep 8
-// waves of 64x32).
+// benchmark the c64 64x64 DMA kernel ran 2.41 ms against 2.98 ms for the
+// register-staged one on a GPU that holds its clock down under load (it moves
+// no slab data through VGPRs, and so draws less power), and 2.44-2.50 ms
+// against 2.35 ms on one that does not.
 bool dma_supported(bool off32, DType t, int method, int tile_id) {
   if (!off32) return false;
   if (t == DType::C64) return tile_id == TILE_64x64 || tile_id == TILE_128x128;
@@ -612,53 +641,29 @@ bool dma_supported(bool off32, DType t, int method, int tile_id) {
 }
 
 hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s) {
-  if (t == DType::C64) {
-    if (method == CPLX_3M) return launch_complex<double, CPLX_3M>(a, tile_id, s);
-    if (method == CPLX_1M) return launch_complex<double, CPLX_1M>(a, tile_id, s);
-    return launch_complex<double, CPLX_NATIVE>(a, tile_id, s);
-  }
-  if (t == DType::C32) {
-    if (method == CPLX_3M) return launch_complex<float, CPLX_3M>(a, tile_id, s);
-    if (method == CPLX_1M) return launch_complex<float, CPLX_1M>(a, tile_id, s);
-    return launch_complex<float, CPLX_NATIVE>(a, tile_id, s);
-  }
-  if (t == DType::F64) {
-    // 128x128: 16 waves of 32x32 with fragment prefetch when A is K-major, 8
-    // waves of 64x32 reading each step as it goes when A is M-major (f64
-    // 4096^3 GEMM, same GPU and run: 55.2 and 52.5 TFLOP/s against 51.2 and
-    // 46.0 for 8 waves with prefetch).
-    if (tile_id == TILE_128x128) {
-      if (a.a_kmajor) {
+  switch (t) {
+    case DType::C64: return single_by_method<double, true>(a, method, tile_id, s);
+    case DType::C32: return single_by_method<float, true>(a, method, tile_id, s);
+    case DType::F64:
+      // 128x128 with K-major A: 16 waves of 32x32 with fragment prefetch (f64
+      // 4096^3 GEMM 55.2 against 51.2 TFLOP/s for the 8-wave layout).
+      if (tile_id == TILE_128x128 && a.a_kmajor) {
         if (a.b_kmajor) return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, true, CPLX_NATIVE>>(a, s);
         return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, false, CPLX_NATIVE>>(a, s);
       }
-      // Two workgroups per CU (LDS allows it) need at most 128 VGPRs.
-      if (a.b_kmajor) return launch_cfg<DCfg<double, false, 128, 128, 16, 64, 32, false, true, CPLX_NATIVE, false, 4>>(a, s);
-      return launch_cfg<DCfg<double, false, 128, 128, 16, 64, 32, false, false, CPLX_NATIVE, false, 4>>(a, s);
-    }
-    if (tile_id == TILE_128x64) return launch_majors<double, false, 128, 64, 16, 64, 32>(a, s);
-    // 8 waves of 32x16 (f64 24^6 with 6 K ranges 31.2 -> 28.4 us against 4
-    // waves of 32x32).
-    return launch_majors<double, false, 64, 64, 16, 32, 16>(a, s);
+      return single_by_method<double, false>(a, method, tile_id, s);
+    case DType::F32: return single_by_method<float, false>(a, method, tile_id, s);
   }
-  if (tile_id == TILE_128x128) return launch_majors<float, false, 128, 128, 16, 64, 32, CPLX_NATIVE, false>(a, s);  // prefetch: 122.7 -> 121.1
-  if (tile_id == TILE_128x64) return launch_majors<float, false, 128, 64, 16, 64, 32>(a, s);
-  return launch_majors<float, false, 64, 64, 16, 32, 32>(a, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_grouped_dma(const GroupItem* items, int n, int64_t total, DType t, int method, int tile_id, int grid,
                               hipStream_t s) {
   switch (t) {
-    case DType::C64:
-      if (method == CPLX_3M) return group_by_tile<double, true, CPLX_3M>(items, n, total, tile_id, grid, s);
-      if (method == CPLX_1M) return group_by_tile<double, true, CPLX_1M>(items, n, total, tile_id, grid, s);
-      return group_by_tile<double, true, CPLX_NATIVE>(items, n, total, tile_id, grid, s);
-    case DType::C32:
-      if (method == CPLX_3M) return group_by_tile<float, true, CPLX_3M>(items, n, total, tile_id, grid, s);
-      if (method == CPLX_1M) return group_by_tile<float, true, CPLX_1M>(items, n, total, tile_id, grid, s);
-      return group_by_tile<float, true, CPLX_NATIVE>(items, n, total, tile_id, grid, s);
-    case DType::F64: return group_by_tile<double, false, CPLX_NATIVE>(items, n, total, tile_id, grid, s);
-    case DType::F32: return group_by_tile<float, false, CPLX_NATIVE>(items, n, total, tile_id, grid, s);
+    case DType::C64: return grouped_by_method<double, true>(items, n, total, method, tile_id, grid, s);
+    case DType::C32: return grouped_by_method<float, true>(items, n, total, method, tile_id, grid, s);
+    case DType::F64: return grouped_by_method<double, false>(items, n, total, method, tile_id, grid, s);
+    case DType::F32: return grouped_by_method<float, false>(items, n, total, method, tile_id, grid, s);
   }
   return hipErrorInvalidValue;
 }
