@@ -49,7 +49,7 @@ typedef struct {
   int32_t precision;      /* TPRIMS_PREC_* */
   int32_t tile;           /* 0: chosen by the planner's tuner; else a tile id (see report) */
   int32_t split_k;        /* 0: chosen by the planner; 1: never split K; n > 1: n K ranges
-                             (storage precision, register-staged tiles; reduced in order) */
+                             (storage precision; partial tiles reduced in split order) */
   int32_t reserved[4];
 } tprims_plan_config;
 
@@ -95,8 +95,11 @@ TAPP_EXPORT TAPP_error tprims_product_report(TAPP_tensor_product plan, tprims_pl
  * Many products of one dtype (each its own plan and shape) bound to their
  * operands and run as one persistent kernel over a device work list. alpha and
  * beta arrays hold one scalar per item (of the dtype). All pointers are device
- * memory. The group owns its work list; create_graph captures the launch into a
- * hipGraph that execute then replays. */
+ * memory. The group owns its work list; use_graph captures the launch into a
+ * hipGraph that execute then replays. (TAPP_execute_batched_product, for
+ * items of one plan, also runs as one launch when every operand is device
+ * memory, the precision is the storage one and no item writes bytes another
+ * item reads or writes; otherwise its items run in order.) */
 typedef intptr_t tprims_group;
 TAPP_EXPORT TAPP_error tprims_group_create(tprims_group* out, TAPP_executor exec, int num_items,
                                            const TAPP_tensor_product* plans, const void* alpha,
