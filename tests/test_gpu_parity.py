@@ -188,3 +188,31 @@ def test_split_k(gpu, dtype, shape):
                           alpha=1.5, beta=-0.5, c=d, cv=vals(d))
     assert ".splitk" in plan.report()["kernel"]
     check(dtype, got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
+def test_split_k_repeats_bitwise(gpu, dtype):
+    """Split K reduces its partial tiles in a fixed order: repeated
+    executions give bitwise identical, oracle-checked results."""
+    import torch
+
+    M, K, N = 96, 2000, 80
+    a = T([M, K], [1, M], [0, 2], conj=dtype.is_complex)
+    b = T([K, N], [1, K], [2, 1])
+    d = T([M, N], [1, M], [0, 1])
+    av, bv, cv = values(dtype, a, 61), values(dtype, b, 62), values(dtype, d, 63)
+    got, want, plan = run(dtype, a, av, b, bv, d, np.zeros(d.storage(), NP[dtype]), alpha=1.25, beta=0.5, c=d, cv=cv)
+    assert ".splitk" in plan.report()["kernel"]
+    check(dtype, got, want)
+    dev = torch.device("cuda:0")
+    ta, tb, tc = (torch.from_numpy(x).to(dev) for x in (av, bv, cv))
+    outs = []
+    for rep in range(4):
+        td = torch.zeros(d.storage(), dtype=ta.dtype, device=dev)
+        plan.execute_raw(1.25, ta.data_ptr(), tb.data_ptr(), 0.5, tc.data_ptr(), td.data_ptr())
+        torch.cuda.synchronize()
+        outs.append(td.cpu().numpy())
+    for o in outs:
+        np.testing.assert_array_equal(o, got)
+
