@@ -55,8 +55,12 @@ __device__ inline void dma_load(__amdgpu_buffer_rsrc_t r, void* dst, uint32_t of
 // PLR_: read each k step's fragments while the previous step's MFMAs run
 // (two fragment sets in registers); off where the accumulators leave no room.
 // MINW_: waves per SIMD the register allocation must allow (1: no bound).
+// VEC_ (real dtypes): every lane's DMA moves 16 bytes, a run of V = 16 / ES
+// elements consecutive in memory along the operand's major index (the
+// planner's ContractArgs::ab_vec); a quarter (f32) or half (f64 word planes:
+// an eighth) of the DMA instructions of the 4-byte form.
 template <typename T, bool CPLX_, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_, int METH_,
-          bool PLR_ = true, int MINW_ = 1>
+          bool PLR_ = true, int MINW_ = 1, bool VEC_ = false>
 struct DCfg {
   typedef T Tv;
   static constexpr bool CPLX = CPLX_, AKM = AKM_, BKM = BKM_, PLR = PLR_;
@@ -71,19 +75,28 @@ struct DCfg {
   // 8-byte elements (no DMA size) arrive as two 4-byte DMAs into separate
   // planes: re and im for c32, the low and high words for f64; f32 and c64
   // elements as one DMA each.
-  static constexpr bool PLANAR = ES == 8;
+  static constexpr bool VEC = VEC_;
+  static_assert(!VEC || !CPLX_, "16-byte runs: real dtypes");
+  static constexpr bool PLANAR = ES == 8 && !VEC;
   static constexpr int PLANES = PLANAR ? 2 : 1;
   typedef typename std::conditional<PLANAR, float, S>::type W;  // LDS element
-  static constexpr int EW = sizeof(W);  // DMA size: 4 or 16
+  static constexpr int EW = sizeof(W);
+  static constexpr int DMAB = VEC ? 16 : EW;  // DMA size: 4 or 16
+  // Run layout (VEC): V elements per 16-byte chunk. K-major parts hold chunk
+  // (row r, k run q) at q * ROWS + r; M/N-major parts hold chunk (row run x,
+  // k) at k * ROWS / V + (x ^ XS(k)), the XOR spreading the four k rows a
+  // fragment read touches over distinct bank groups.
+  static constexpr int V = VEC ? 16 / ES : 1;
+  static constexpr int XQ = V == 4 ? 4 : 8, XR = V == 4 ? 4 : 2;
   static constexpr int RPI = 64 / BK;   // rows per K-major block
   static constexpr int PADB = EW == 16 ? 1 : 4;
   static constexpr int SW = EW == 16 ? 4 : 0;
   static constexpr int PADM = EW == 16 ? 0 : 16;
-  static constexpr int A_ELEMS = AKM ? (BM / RPI) * (64 + PADB) : BK * (BM + PADM);
-  static constexpr int B_ELEMS = BKM ? (BN / RPI) * (64 + PADB) : BK * (BN + PADM);
+  static constexpr int A_ELEMS = VEC ? BM * BK : AKM ? (BM / RPI) * (64 + PADB) : BK * (BM + PADM);
+  static constexpr int B_ELEMS = VEC ? BN * BK : BKM ? (BN / RPI) * (64 + PADB) : BK * (BN + PADM);
   static constexpr int PLANE = A_ELEMS + B_ELEMS;  // LDS elements of one plane
   static constexpr int STAGE = PLANES * PLANE;     // LDS elements of one stage
-  static constexpr int A_INSTR = BM * BK / 64, B_INSTR = BN * BK / 64;
+  static constexpr int A_INSTR = BM * BK / (64 * V), B_INSTR = BN * BK / (64 * V);
   static constexpr int JA = A_INSTR / NW, JB = B_INSTR / NW;  // DMA instructions per wave per slab
   static constexpr int MI = WM / 16, NI = WN / 16;
   // 1m runs the real GEMM [Dre; Dim] = [[Ar, -Ai], [Ai, Ar]] [Br; Bi]: a
@@ -102,7 +115,9 @@ struct DCfg {
   static constexpr int EPI_ROWS = (size_t)BM * EPI_ROW_BYTES <= LDS_MAIN ? BM : WM;
   static constexpr size_t LDS_EPI = (size_t)EPI_ROWS * EPI_ROW_BYTES;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
-  static_assert(EW == 4 || EW == 16, "LDS DMA moves 4 or 16 bytes per lane");
+  static_assert(DMAB == 4 || DMAB == 16, "LDS DMA moves 4 or 16 bytes per lane");
+  static_assert(!VEC || (BM % 64 == 0 && BN % 64 == 0 && BM / V >= 4 * XQ && BN / V >= 4 * XQ && BK % V == 0),
+                "run layout: whole 64-row instructions, room for the XOR");
   static_assert(64 % BK == 0 && BM % RPI == 0 && BN % RPI == 0, "K-major blocks");
   static_assert(BM % 64 == 0 || AKM, "M-major rows are whole DMA instructions");
   static_assert(BN % 64 == 0 || BKM, "N-major rows are whole DMA instructions");
@@ -113,6 +128,8 @@ struct DCfg {
   // LDS slot of element (row r, k) of a stage's A (rows BM) or B (rows BN) part.
   template <bool KM, int ROWS>
   __device__ static inline int slot(int r, int k) {
+    if constexpr (VEC && KM) return ((k / V) * ROWS + r) * V + k % V;
+    if constexpr (VEC) return (k * (ROWS / V) + ((r / V) ^ (XQ * (k % XR)))) * V + r % V;
     if constexpr (KM) return (r / RPI) * (64 + PADB) + (r % RPI) * BK + (k ^ (SW * (r % RPI)));
     else return k * (ROWS + PADM) + r;
   }
@@ -148,7 +165,14 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   for (int t = 0; t < C::JA; t++) {
     const int j = wave + NW * t;
     int r, kl;
-    if constexpr (C::AKM) {
+    if constexpr (C::VEC && C::AKM) {
+      r = (j % (BM / 64)) * 64 + lane;
+      kl = C::V * (j / (BM / 64));
+    } else if constexpr (C::VEC) {
+      const int q = j * 64 + lane, k = q / (BM / C::V);
+      r = ((q % (BM / C::V)) ^ (C::XQ * (k % C::XR))) * C::V;
+      kl = k;
+    } else if constexpr (C::AKM) {
       r = j * C::RPI + lane / BK;
       kl = (lane % BK) ^ (C::SW * (lane / BK));
     } else {
@@ -163,7 +187,14 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   for (int t = 0; t < C::JB; t++) {
     const int j = wave + NW * t;
     int r, kl;
-    if constexpr (C::BKM) {
+    if constexpr (C::VEC && C::BKM) {
+      r = (j % (BN / 64)) * 64 + lane;
+      kl = C::V * (j / (BN / 64));
+    } else if constexpr (C::VEC) {
+      const int q = j * 64 + lane, k = q / (BN / C::V);
+      r = ((q % (BN / C::V)) ^ (C::XQ * (k % C::XR))) * C::V;
+      kl = k;
+    } else if constexpr (C::BKM) {
       r = j * C::RPI + lane / BK;
       kl = (lane % BK) ^ (C::SW * (lane / BK));
     } else {
@@ -177,10 +208,12 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   // LDS destination of instruction t of this wave within a stage.
   auto a_dst = [&](int t) {
     const int j = wave + NW * t;
+    if constexpr (C::VEC) return j * 64 * C::V;
     return C::AKM ? j * (64 + C::PADB) : (j / (BM / 64)) * (BM + C::PADM) + (j % (BM / 64)) * 64;
   };
   auto b_dst = [&](int t) {
     const int j = wave + NW * t;
+    if constexpr (C::VEC) return C::A_ELEMS + j * 64 * C::V;
     return C::A_ELEMS + (C::BKM ? j * (64 + C::PADB) : (j / (BN / 64)) * (BN + C::PADM) + (j % (BN / 64)) * 64);
   };
 
@@ -192,15 +225,15 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
 #pragma unroll
     for (int t = 0; t < C::JB; t++) kb[t] = p.bk32[k0 + b_kl[t]];
   };
-  // Plane c of an element holds its bytes [4 c, 4 c + EW) (re, then im for c32).
+  // Plane c of an element holds its bytes [4 c, 4 c + 4) (re, then im for c32).
   auto issue = [&](int st, const uint32_t* ka, const uint32_t* kb) {
 #pragma unroll
     for (int c = 0; c < C::PLANES; c++) {
       W* stage = lds + st * C::STAGE + c * C::PLANE;
 #pragma unroll
-      for (int t = 0; t < C::JA; t++) dma_load<C::EW>(rsA, stage + a_dst(t), a_row[t] + ka[t] + 4 * c);
+      for (int t = 0; t < C::JA; t++) dma_load<C::DMAB>(rsA, stage + a_dst(t), a_row[t] + ka[t] + 4 * c);
 #pragma unroll
-      for (int t = 0; t < C::JB; t++) dma_load<C::EW>(rsB, stage + b_dst(t), b_row[t] + kb[t] + 4 * c);
+      for (int t = 0; t < C::JB; t++) dma_load<C::DMAB>(rsB, stage + b_dst(t), b_row[t] + kb[t] + 4 * c);
     }
   };
 
@@ -528,14 +561,14 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH, bool PLR, int MINW>
+template <typename T, bool CPLX, int BM, int BN, int BK, int WM, int WN, int METH, bool PLR, int MINW, bool VEC = false>
 hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
   if (a.a_kmajor) {
-    if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH, PLR, MINW>>(a, s);
-    return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH, PLR, MINW>>(a, s);
+    if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, true, METH, PLR, MINW, VEC>>(a, s);
+    return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, true, false, METH, PLR, MINW, VEC>>(a, s);
   }
-  if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH, PLR, MINW>>(a, s);
-  return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR, MINW>>(a, s);
+  if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, true, METH, PLR, MINW, VEC>>(a, s);
+  return launch_cfg<DCfg<T, CPLX, BM, BN, BK, WM, WN, false, false, METH, PLR, MINW, VEC>>(a, s);
 }
 
 // The wave layout of every DMA tile (BM x BN over waves of WM x WN, fragment
@@ -584,6 +617,8 @@ struct Single {
   hipStream_t s;
   template <int BM, int BN, int WM, int WN, bool PLR, int MINW>
   hipError_t go() const {
+    if constexpr (!CPLX)
+      if (a.ab_vec) return launch_majors<T, CPLX, BM, BN, 16, WM, WN, METH, PLR, MINW, true>(a, s);
     return launch_majors<T, CPLX, BM, BN, 16, WM, WN, METH, PLR, MINW>(a, s);
   }
 };
@@ -648,6 +683,10 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int t
       // 128x128 with K-major A: 16 waves of 32x32 with fragment prefetch (f64
       // 4096^3 GEMM 55.2 against 51.2 TFLOP/s for the 8-wave layout).
       if (tile_id == TILE_128x128 && a.a_kmajor) {
+        if (a.ab_vec) {
+          if (a.b_kmajor) return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, true, CPLX_NATIVE, true, 1, true>>(a, s);
+          return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, false, CPLX_NATIVE, true, 1, true>>(a, s);
+        }
         if (a.b_kmajor) return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, true, CPLX_NATIVE>>(a, s);
         return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, false, CPLX_NATIVE>>(a, s);
       }

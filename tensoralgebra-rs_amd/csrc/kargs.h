@@ -46,7 +46,12 @@ struct ContractArgs {
   // entry's base) is nonnegative and below 2^30: gathers use 32-bit buffer
   // offsets.
   int32_t off32;
-  int32_t pad0;
+  // Real dtypes with off32: A and B gather in 16-byte runs. Along its major
+  // index (K, or M for A / N for B) each operand's scatter consists of runs
+  // of 16 / es consecutive elements starting at multiples of 16 / es, its
+  // extent there is a multiple of 16 / es, and every run start, k offset,
+  // row offset and batch offset is a multiple of 16 bytes, as are A and B.
+  int32_t ab_vec;
   // With off32: the K scatters as 32-bit byte offsets, entries from K up to a
   // multiple of 64 holding the out-of-range sentinel 0x40000000.
   const uint32_t* ak32;

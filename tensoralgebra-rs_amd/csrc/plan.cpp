@@ -305,6 +305,39 @@ void configure(Plan* plan, const PlanConfig& cfg) {
       p.bk32[k] = (uint32_t)(p.bk[k] * es);
     }
   }
+  // 16-byte runs (ContractArgs::ab_vec): q = 16 / es consecutive elements
+  // along each operand's major index, every offset a multiple of q.
+  p.ab_vec = false;
+  if (p.off32 && !is_complex(p.prob.dtype)) {
+    const int64_t q = 16 / es;
+    auto runs = [&](const std::vector<int64_t>& v, int64_t n) {
+      if (n % q != 0) return false;
+      for (int64_t i = 0; i < n; i += q)
+        for (int64_t j = 1; j < q; j++)
+          if (v[i + j] != v[i] + j) return false;
+      return true;
+    };
+    auto mult = [&](const std::vector<int64_t>& v, int64_t n) {
+      for (int64_t i = 0; i < n; i++)
+        if (v[i] % q != 0) return false;
+      return true;
+    };
+    auto operand = [&](bool kmajor, const std::vector<int64_t>& rv, int64_t rn, const std::vector<int64_t>& kv,
+                       const std::vector<int64_t>& hv) {
+      if (!mult(hv, p.H)) return false;
+      if (kmajor) {
+        if (!runs(kv, p.K) || !mult(rv, rn)) return false;
+        for (int64_t k = 0; k < p.K; k += q)
+          if (kv[k] % q != 0) return false;
+        return true;
+      }
+      if (!runs(rv, rn) || !mult(kv, p.K)) return false;
+      for (int64_t i = 0; i < rn; i += q)
+        if (rv[i] % q != 0) return false;
+      return true;
+    };
+    p.ab_vec = operand(p.a_kmajor, p.am, p.M, p.ak, p.ha) && operand(p.b_kmajor, p.bn, p.N, p.bk, p.hb);
+  }
 }
 
 Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
@@ -418,6 +451,7 @@ ContractArgs Plan::args(int device, const void* A, const void* B, const void* C,
   a.a_kvec = a_kvec && (uintptr_t)A % 16 == 0;
   a.b_kvec = b_kvec && (uintptr_t)B % 16 == 0;
   a.off32 = off32;
+  a.ab_vec = ab_vec && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0;
   a.ak32 = s.ak32;
   a.bk32 = s.bk32;
   a.h_affine = h_ax.size() == 1;
@@ -448,7 +482,7 @@ std::string Plan::kernel_name() const {
   const TileCfg* t = tile_cfg(fallback ? TILE_64x64 : tile);
   if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(cfg.precision == PREC_FP8 ? 128 : t->bk);
   if (cfg.precision == PREC_STORAGE && !fallback && dma_supported(off32, prob.dtype, cfg.complex_method, tile))
-    s += ".dma";
+    s += ab_vec ? ".dma16" : ".dma";
   if (ksplit > 1) s += ".splitk" + std::to_string(ksplit);
   return s;
 }

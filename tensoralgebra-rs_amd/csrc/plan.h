@@ -47,6 +47,7 @@ struct Plan {
   // checked per call).
   bool a_kvec = false, b_kvec = false;
   bool off32 = false;  // ContractArgs::off32
+  bool ab_vec = false;  // host half of ContractArgs::ab_vec (pointer alignment checked per call)
   // Split K (ContractArgs::ksplit, kslabs): 1, or the number of K ranges a
   // tile's work is divided into when the tiles alone would leave most of the
   // GPU idle; the partials are reduced in a fixed order.

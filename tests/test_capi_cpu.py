@@ -322,7 +322,7 @@ def test_split_k_choice_and_override():
     e = 24
     sp = [1, e, e * e, e ** 3]
     args = (T([e] * 4, sp, [0, 4, 1, 5]), T([e] * 4, sp, [5, 2, 4, 3]), T([e] * 4, sp, [0, 1, 2, 3]))
-    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.64x64x16.dma.splitk3"
+    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.64x64x16.dma16.splitk3"
     assert ".splitk" not in make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=1)).report()["kernel"]
     assert make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=5)).report()["kernel"].endswith(".splitk5")
     with pytest.raises(tp.TprimsError):
@@ -342,7 +342,9 @@ def test_tile_choice_follows_the_busiest_cu_estimate():
 
     for dt in (tp.DType.F32, tp.DType.F64, tp.DType.C32, tp.DType.C64):
         big = gemm(dt, 4096, 4096, 256)
-        assert big["tile"] == 4 and big["kernel"].endswith("128x128x16.dma"), big["kernel"]
+        # real dtypes gather 16-byte runs (column-major operands qualify)
+        suffix = "128x128x16.dma16" if dt in (tp.DType.F32, tp.DType.F64) else "128x128x16.dma"
+        assert big["tile"] == 4 and big["kernel"].endswith(suffix), big["kernel"]
     # 512 x 512 x 16 batches fill the GPU with 128x128 tiles (the headline shape)
     assert gemm(tp.DType.C64, 512, 512, 64, 16)["tile"] == 4
     # one 64x64 output: a 64x64 tile on 4 SIMDs, not a 128x128 one

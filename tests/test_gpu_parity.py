@@ -191,6 +191,53 @@ def test_split_k(gpu, dtype, shape):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.F64], ids=lambda d: d.name)
+@pytest.mark.parametrize("tile", [1, 3, 4])
+@pytest.mark.parametrize("majors", ["kk", "kn", "mk", "mn"])
+@pytest.mark.parametrize("split", [1, 3])
+def test_dma_16_byte_runs(gpu, dtype, tile, majors, split):
+    """Real operands whose gathers are 16-byte runs (4 f32 or 2 f64
+    consecutive along the major index, every offset a multiple of 16 bytes)
+    load whole runs per lane: every tile, both majors of A and B, a batch
+    index, edges that do not fill a tile, K split or not."""
+    M, K, N, H = 148, 340, 136, 2
+    a = T([K, M, H], [1, K, K * M], [2, 0, 3]) if majors[0] == "k" else T([M, K, H], [1, M, M * K], [0, 2, 3])
+    b = T([K, N, H], [1, K, K * N], [2, 1, 3]) if majors[1] == "k" else T([N, K, H], [1, N, N * K], [1, 2, 3])
+    d = T([M, N, H], [1, M, M * N], [0, 1, 3])
+    av, bv, cv = values(dtype, a, 61), values(dtype, b, 62), values(dtype, d, 63)
+    got, want, plan = run(dtype, a, av, b, bv, d, np.zeros(d.storage(), NP[dtype]), alpha=0.75, beta=-0.5, c=d,
+                          cv=cv, config=tp.PlanConfig(tile=tile, split_k=split))
+    rep = plan.report()
+    assert ".dma16" in rep["kernel"] and rep["a_k_major"] == (majors[0] == "k") and \
+        rep["b_k_major"] == (majors[1] == "k"), rep
+    check(dtype, got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.F64], ids=lambda d: d.name)
+@pytest.mark.parametrize("case", ["k_not_multiple", "odd_row_offsets", "short_runs"])
+def test_dma_without_16_byte_runs(gpu, dtype, case):
+    """Layouts that do not qualify for 16-byte runs keep the 4-byte gathers
+    and still match the oracle."""
+    M, K, N = 132, 203, 68
+    if case == "k_not_multiple":  # K-major A with K odd
+        a, b = T([K, M], [1, K], [2, 0]), T([K, N], [1, K], [2, 1])
+    elif case == "odd_row_offsets":  # K-major A whose rows start at odd elements (leading dimension K + 1)
+        K = 204
+        a, b = T([K, M], [1, K + 1], [2, 0]), T([K, N], [1, K], [2, 1])
+    else:  # M-major A in runs of 2 (stride-2 pairs): runs too short for f32
+        a = T([2, M // 2, K], [1, 4, 2 * M], [0, 4, 2])
+        b = T([K, N], [1, K], [2, 1])
+    d = T([M, N], [1, M], [0, 1]) if case != "short_runs" else T([2, M // 2, N], [1, 2, M], [0, 4, 1])
+    av, bv = values(dtype, a, 71), values(dtype, b, 72)
+    got, want, plan = run(dtype, a, av, b, bv, d, np.zeros(d.storage(), NP[dtype]), alpha=1.25,
+                          config=tp.PlanConfig(split_k=1))
+    kern = plan.report()["kernel"]
+    assert ".dma16" not in kern or (case == "short_runs" and dtype == tp.DType.F64), kern
+    check(dtype, got, want)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
 def test_split_k_repeats_bitwise(gpu, dtype):
     """Split K reduces its partial tiles in a fixed order: repeated
