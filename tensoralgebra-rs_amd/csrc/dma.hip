@@ -583,8 +583,8 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
 //    3m without prefetch: 63 against 54).
 //  - c32 128x128: 16 waves of 32x32 (native 101 -> 109.5 and 3m 105 -> 119
 //    without prefetch, 1m 89 -> 97 with it).
-//  - f64 128x128 (grouped, or M-major A): 8 waves of 64x32 without prefetch,
-//    at most 128 VGPRs so two workgroups share a CU; f64 64x64: 8 waves of
+//  - f64 128x128 (grouped, M-major A, or any 16-byte run layout): 8 waves of
+//    64x32 without prefetch, at most 128 VGPRs so two workgroups share a CU; f64 64x64: 8 waves of
 //    32x16 (f64 24^6 31.2 -> 28.4 us against 4 of 32x32).
 //  - f32 128x128: 8 waves of 64x32 without prefetch (122.7 against 121.1).
 template <typename T, bool CPLX, int METH, class Act>
@@ -680,13 +680,12 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int t
     case DType::C64: return single_by_method<double, true>(a, method, tile_id, s);
     case DType::C32: return single_by_method<float, true>(a, method, tile_id, s);
     case DType::F64:
-      // 128x128 with K-major A: 16 waves of 32x32 with fragment prefetch (f64
-      // 4096^3 GEMM 55.2 against 51.2 TFLOP/s for the 8-wave layout).
-      if (tile_id == TILE_128x128 && a.a_kmajor) {
-        if (a.ab_vec) {
-          if (a.b_kmajor) return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, true, CPLX_NATIVE, true, 1, true>>(a, s);
-          return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, false, CPLX_NATIVE, true, 1, true>>(a, s);
-        }
+      // 128x128 with K-major A in 4-byte gathers: 16 waves of 32x32 with
+      // fragment prefetch (f64 4096^3 GEMM 55.2 against 51.2 TFLOP/s for the
+      // 8-wave layout). With 16-byte runs the 8-wave layout of the table
+      // serves every major (K-major A 56.3 -> 62.8 TFLOP/s; M-major A 59.4
+      // against 55.9 for 16 waves).
+      if (tile_id == TILE_128x128 && a.a_kmajor && !a.ab_vec) {
         if (a.b_kmajor) return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, true, CPLX_NATIVE>>(a, s);
         return launch_cfg<DCfg<double, false, 128, 128, 16, 32, 32, true, false, CPLX_NATIVE>>(a, s);
       }
