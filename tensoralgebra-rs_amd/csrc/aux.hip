@@ -196,6 +196,29 @@ __global__ void k_splitk_reduce(ContractArgs p) {
   }
 }
 
+// The same over a two-dimensional grid: blockIdx.y is one line of the
+// workspace (h and the slower output index), so a lane finds its element
+// without 64-bit divisions (the line's split is uniform across the block).
+template <typename T, bool CPLX>
+__global__ void k_splitk_reduce_lines(ContractArgs p) {
+  typedef Store<T, CPLX> St;
+  typedef typename St::S S;
+  const int64_t total = p.M * p.N * p.H;
+  const int64_t inner = p.d_mmajor ? p.M : p.N, outer = p.d_mmajor ? p.N : p.M;
+  const int64_t line = blockIdx.y, o = line % outer, h = line / outer;
+  const S* W = (const S*)p.work + line * inner;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < inner; i += (int64_t)gridDim.x * blockDim.x) {
+    T sr = 0, si = 0;
+    for (int s = 0; s < p.ksplit; s++) {
+      const S v = W[(int64_t)s * total + i];
+      sr += St::re(v);
+      si += St::im(v);
+    }
+    if (p.d_mmajor) dev::store_element<T, CPLX>(p, h, i, o, sr, si);
+    else dev::store_element<T, CPLX>(p, h, o, i, sr, si);
+  }
+}
+
 struct ZeroLayout {
   int rank;
   int64_t dims[8];
@@ -301,6 +324,19 @@ hipError_t launch_splitk_reduce(const ContractArgs& a, DType t, hipStream_t s) {
   const int64_t total = a.M * a.N * a.H;
   if (total == 0) return hipSuccess;
   const int threads = 256;
+  const int64_t inner = a.d_mmajor ? a.M : a.N, lines = total / inner;
+  if (lines <= 65535) {  // one grid row per workspace line
+    int64_t bx = (inner + threads - 1) / threads;
+    if (bx > 1024) bx = 1024;
+    const dim3 grid((unsigned)bx, (unsigned)lines);
+    switch (t) {
+      case DType::F64: hipLaunchKernelGGL((k_splitk_reduce_lines<double, false>), grid, dim3(threads), 0, s, a); break;
+      case DType::F32: hipLaunchKernelGGL((k_splitk_reduce_lines<float, false>), grid, dim3(threads), 0, s, a); break;
+      case DType::C64: hipLaunchKernelGGL((k_splitk_reduce_lines<double, true>), grid, dim3(threads), 0, s, a); break;
+      case DType::C32: hipLaunchKernelGGL((k_splitk_reduce_lines<float, true>), grid, dim3(threads), 0, s, a); break;
+    }
+    return hipGetLastError();
+  }
   int64_t blocks = (total + threads - 1) / threads;
   if (blocks > 65536) blocks = 65536;
   switch (t) {

@@ -169,15 +169,17 @@ def test_dma_complex_schedules_on_large_k(gpu, dtype, method, tile):
 @pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
 @pytest.mark.parametrize("shape", [(40, 3000, 24, 1), (100, 700, 90, 2), (576, 576, 576, 1)],
                          ids=lambda s: "x".join(map(str, s)))
-def test_split_k(gpu, dtype, shape):
+@pytest.mark.parametrize("d_major", ["m", "n"])
+def test_split_k(gpu, dtype, shape, d_major):
     """Few output tiles and a long K: the planner splits K (fixed-order
     reduction of the partial tiles, then the write-back with alpha, beta and
-    conj) and the result matches the oracle."""
+    conj) and the result matches the oracle, with D (and C) contiguous along
+    M or along N."""
     from harness import values
     M, K, N, H = shape
     a = T([M, K, H], [1, M, M * K], [0, 2, 3], conj=dtype.is_complex)
     b = T([K, N, H], [1, K, K * N], [2, 1, 3])
-    d = T([M, N, H], [1, M, M * N], [0, 1, 3])
+    d = T([M, N, H], [1, M, M * N] if d_major == "m" else [N, 1, M * N], [0, 1, 3])
     rng = np.random.default_rng(3)
     def vals(t):
         v = rng.uniform(-1, 1, t.storage())
