@@ -51,6 +51,11 @@ def parse():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-config", default=None,
+                    choices=["f64_24", "f64_gemm4096", "transform", "gemm4096", "grouped", "tapp_batched",
+                             "elementwise", "vjp"],
+                    help="run one benchmark contraction of the configs table (with --steps and --warmup), print "
+                         "its JSON object and exit (for counter passes)")
     return ap.parse_args()
 
 
@@ -388,6 +393,18 @@ def cpu_baseline(args):
     return packed.bench_complex_sample(dtype=args.dtype, seconds=10.0)
 
 
+def config_table(torch, tp, np, ex, stream, steps):
+    """The benchmark contractions beside the headline, by name."""
+    return (("f64_24", lambda: config_f64_24(torch, tp, np, ex, stream, steps)),
+            ("f64_gemm4096", lambda: config_f64_gemm(torch, tp, np, ex, stream, steps)),
+            ("transform", lambda: config_precision(torch, tp, np, ex, stream, steps, "transform")),
+            ("gemm4096", lambda: config_precision(torch, tp, np, ex, stream, steps, "gemm")),
+            ("grouped", lambda: config_grouped(torch, tp, np, ex, stream, steps)),
+            ("tapp_batched", lambda: config_tapp_batched(torch, tp, np, ex, stream, steps)),
+            ("elementwise", lambda: config_elementwise(torch, tp, np, ex, stream, steps)),
+            ("vjp", lambda: config_vjp(torch, tp, np, ex, stream, steps)))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -403,6 +420,10 @@ def main():
         return
     stream = torch.cuda.Stream()
     ex = tp.Executor(0, stream.cuda_stream, asynchronous=True)
+    table = config_table(torch, tp, np, ex, stream, args.steps)
+    if args.only_config:
+        print(json.dumps({"config": args.only_config, **dict(table)[args.only_config]()}))
+        return
     w = Complex(torch, tp, args.dtype, args.method, args.precision)
     t0 = time.perf_counter()
     ms = timed(torch, lambda: w.run(ex), stream, args.steps, args.warmup)
@@ -470,14 +491,7 @@ def main():
             out["split_bf16x3_vs_fp32"] = round(s3 / f32, 3)
     if not args.no_configs:
         cfgs = {}
-        for name, fn in (("f64_24", lambda: config_f64_24(torch, tp, np, ex, stream, args.steps)),
-                         ("f64_gemm4096", lambda: config_f64_gemm(torch, tp, np, ex, stream, args.steps)),
-                         ("transform", lambda: config_precision(torch, tp, np, ex, stream, args.steps, "transform")),
-                         ("gemm4096", lambda: config_precision(torch, tp, np, ex, stream, args.steps, "gemm")),
-                         ("grouped", lambda: config_grouped(torch, tp, np, ex, stream, args.steps)),
-                         ("tapp_batched", lambda: config_tapp_batched(torch, tp, np, ex, stream, args.steps)),
-                         ("elementwise", lambda: config_elementwise(torch, tp, np, ex, stream, args.steps)),
-                         ("vjp", lambda: config_vjp(torch, tp, np, ex, stream, args.steps))):
+        for name, fn in table:
             try:
                 cfgs[name] = fn()
             except Exception as e:

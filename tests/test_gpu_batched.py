@@ -84,6 +84,29 @@ def test_batched_items_match_the_oracle(gpu, dtype, shape):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.F64], ids=lambda d: d.name)
+@pytest.mark.parametrize("gap", [3, 16], ids=["origins_misaligned", "origins_aligned"])
+def test_batched_items_in_16_byte_runs(gpu, dtype, gap):
+    """A layout that gathers in 16-byte runs: the fused launch keeps the run
+    gathers only when every item's origin is 16-byte aligned (gap 16), and
+    falls back to 4-byte gathers otherwise (gap 3); both match the oracle."""
+    import torch
+
+    a, b, d = T([64, 48], [1, 64], [0, 2]), T([48, 32], [1, 48], [2, 1]), T([64, 32], [1, 64], [0, 1])
+    plan = make_plan(dtype, a, b, d, d)
+    assert ".dma16" in plan.report()["kernel"], plan.report()
+    alpha, beta = 1.25, -0.5
+    items = _items(torch, dtype, a, b, d, 5, seed=90, gap=gap)
+    plan.execute_batched_raw(alpha, [it["a"] for it in items], [it["b"] for it in items], beta,
+                             [it["c"] for it in items], [it["d"] for it in items])
+    torch.cuda.synchronize()
+    for it in items:
+        want = _want(dtype, a, it["a_np"](), b, it["b_np"](), d, np.zeros(d.storage(), NP[dtype]), alpha, beta,
+                     it["c_np"]())
+        check(dtype, it["d_np"](), want)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("dtype", [tp.DType.F64, tp.DType.C32], ids=lambda d: d.name)
 def test_batched_items_sharing_one_output_accumulate_in_order(gpu, dtype):
     """Every item accumulates into the same D (C = D, beta = 1): the items
