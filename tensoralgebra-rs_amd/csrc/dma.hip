@@ -55,10 +55,10 @@ __device__ inline void dma_load(__amdgpu_buffer_rsrc_t r, void* dst, uint32_t of
 // PLR_: read each k step's fragments while the previous step's MFMAs run
 // (two fragment sets in registers); off where the accumulators leave no room.
 // MINW_: waves per SIMD the register allocation must allow (1: no bound).
-// VEC_ (real dtypes): every lane's DMA moves 16 bytes, a run of V = 16 / ES
+// VEC_ (f32, f64, c32): every lane's DMA moves 16 bytes, a run of V = 16 / ES
 // elements consecutive in memory along the operand's major index (the
-// planner's ContractArgs::ab_vec); a quarter (f32) or half (f64 word planes:
-// an eighth) of the DMA instructions of the 4-byte form.
+// planner's ContractArgs::ab_vec); a quarter of the DMA instructions of the
+// 4-byte form for f32, an eighth for f64 and c32 (no word or re/im planes).
 template <typename T, bool CPLX_, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_, int METH_,
           bool PLR_ = true, int MINW_ = 1, bool VEC_ = false>
 struct DCfg {
@@ -76,7 +76,7 @@ struct DCfg {
   // planes: re and im for c32, the low and high words for f64; f32 and c64
   // elements as one DMA each.
   static constexpr bool VEC = VEC_;
-  static_assert(!VEC || !CPLX_, "16-byte runs: real dtypes");
+  static_assert(!VEC || ES == 4 || ES == 8, "16-byte runs: f32, f64, c32");
   static constexpr bool PLANAR = ES == 8 && !VEC;
   static constexpr int PLANES = PLANAR ? 2 : 1;
   typedef typename std::conditional<PLANAR, float, S>::type W;  // LDS element
@@ -106,7 +106,9 @@ struct DCfg {
   // Accumulators: real and 1m 1; c64 native 2 (re, im; the f64 MFMA negates
   // -im*im for free); c32 native 3 (re*re, im*im subtracted at the end,
   // cross); 3m 3 (re*re, im*im, (re+im)*(re+im)).
-  static constexpr int ACCS = !CPLX || METH == CPLX_1M ? 1 : (PLANAR || METH == CPLX_3M ? 3 : 2);
+  // c32 native keeps three accumulators: the f32 MFMA has no negate flag.
+  static constexpr bool C3 = CPLX && sizeof(T) == 4 && METH == CPLX_NATIVE;
+  static constexpr int ACCS = !CPLX || METH == CPLX_1M ? 1 : (C3 || METH == CPLX_3M ? 3 : 2);
   static constexpr int LDD = BN + 1;
   static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * EW;
   // The finished tile is staged whole when it fits in the main loop's LDS,
@@ -366,7 +368,7 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
     } else {
       // c64: acc0 += ar br - ai bi, acc1 += ar bi + ai br. c32: acc0 += ar br,
       // acc1 += ai bi (subtracted at the end), acc2 += ar bi + ai br.
-      constexpr int XI = C::PLANAR ? 2 : 1;
+      constexpr int XI = C::C3 ? 2 : 1;
 #pragma unroll
       for (int i = 0; i < C::MI; i++)
 #pragma unroll
@@ -378,7 +380,7 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
       for (int i = 0; i < C::MI; i++)
 #pragma unroll
         for (int j = 0; j < C::NI; j++) {
-          if constexpr (C::PLANAR) acc[1][i][j] = MM::run(f.a[2 * i + 1], f.b[2 * j + 1], acc[1][i][j]);
+          if constexpr (C::C3) acc[1][i][j] = MM::run(f.a[2 * i + 1], f.b[2 * j + 1], acc[1][i][j]);
           else acc[0][i][j] = MM::run_neg_a(f.a[2 * i + 1], f.b[2 * j + 1], acc[0][i][j]);
           acc[XI][i][j] = MM::run(f.a[2 * i + 1], f.b[2 * j], acc[XI][i][j]);
         }
@@ -472,7 +474,7 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
             if constexpr (C::METH == CPLX_3M) {
               Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
               Dim[row * C::LDD + col] = acc[2][i][j][q] - acc[0][i][j][q] - acc[1][i][j][q];
-            } else if constexpr (C::CPLX && C::PLANAR) {
+            } else if constexpr (C::C3) {
               Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
               Dim[row * C::LDD + col] = acc[2][i][j][q];
             } else {
@@ -617,7 +619,7 @@ struct Single {
   hipStream_t s;
   template <int BM, int BN, int WM, int WN, bool PLR, int MINW>
   hipError_t go() const {
-    if constexpr (!CPLX)
+    if constexpr (!CPLX || sizeof(T) == 4)
       if (a.ab_vec) return launch_majors<T, CPLX, BM, BN, 16, WM, WN, METH, PLR, MINW, true>(a, s);
     return launch_majors<T, CPLX, BM, BN, 16, WM, WN, METH, PLR, MINW>(a, s);
   }

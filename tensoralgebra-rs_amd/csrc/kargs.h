@@ -46,7 +46,7 @@ struct ContractArgs {
   // entry's base) is nonnegative and below 2^30: gathers use 32-bit buffer
   // offsets.
   int32_t off32;
-  // Real dtypes with off32: A and B gather in 16-byte runs. Along its major
+  // f32, f64, c32 with off32: A and B gather in 16-byte runs. Along its major
   // index (K, or M for A / N for B) each operand's scatter consists of runs
   // of 16 / es consecutive elements starting at multiples of 16 / es, its
   // extent there is a multiple of 16 / es, and every run start, k offset,

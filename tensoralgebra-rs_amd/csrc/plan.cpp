@@ -308,7 +308,7 @@ void configure(Plan* plan, const PlanConfig& cfg) {
   // 16-byte runs (ContractArgs::ab_vec): q = 16 / es consecutive elements
   // along each operand's major index, every offset a multiple of q.
   p.ab_vec = false;
-  if (p.off32 && !is_complex(p.prob.dtype)) {
+  if (p.off32 && es < 16) {  // f32, f64, c32 (a c64 element is already one 16-byte DMA)
     const int64_t q = 16 / es;
     auto runs = [&](const std::vector<int64_t>& v, int64_t n) {
       if (n % q != 0) return false;

@@ -342,8 +342,8 @@ def test_tile_choice_follows_the_busiest_cu_estimate():
 
     for dt in (tp.DType.F32, tp.DType.F64, tp.DType.C32, tp.DType.C64):
         big = gemm(dt, 4096, 4096, 256)
-        # real dtypes gather 16-byte runs (column-major operands qualify)
-        suffix = "128x128x16.dma16" if dt in (tp.DType.F32, tp.DType.F64) else "128x128x16.dma"
+        # f32, f64 and c32 gather 16-byte runs (column-major operands qualify)
+        suffix = "128x128x16.dma" if dt == tp.DType.C64 else "128x128x16.dma16"
         assert big["tile"] == 4 and big["kernel"].endswith(suffix), big["kernel"]
     # 512 x 512 x 16 batches fill the GPU with 128x128 tiles (the headline shape)
     assert gemm(tp.DType.C64, 512, 512, 64, 16)["tile"] == 4

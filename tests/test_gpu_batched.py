@@ -84,7 +84,7 @@ def test_batched_items_match_the_oracle(gpu, dtype, shape):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.F64], ids=lambda d: d.name)
+@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.F64, tp.DType.C32], ids=lambda d: d.name)
 @pytest.mark.parametrize("gap", [3, 16], ids=["origins_misaligned", "origins_aligned"])
 def test_batched_items_in_16_byte_runs(gpu, dtype, gap):
     """A layout that gathers in 16-byte runs: the fused launch keeps the run

@@ -192,27 +192,37 @@ def test_split_k(gpu, dtype, shape, d_major):
     check(dtype, got, want)
 
 
+_RUN_VARIANTS = [(tp.DType.F32, tp.ComplexMethod.NATIVE), (tp.DType.F64, tp.ComplexMethod.NATIVE),
+                 (tp.DType.C32, tp.ComplexMethod.NATIVE), (tp.DType.C32, tp.ComplexMethod.ONE_M),
+                 (tp.DType.C32, tp.ComplexMethod.THREE_M)]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.F64], ids=lambda d: d.name)
+@pytest.mark.parametrize("dtype,method", _RUN_VARIANTS, ids=lambda v: v.name)
 @pytest.mark.parametrize("tile", [1, 3, 4])
 @pytest.mark.parametrize("majors", ["kk", "kn", "mk", "mn"])
 @pytest.mark.parametrize("split", [1, 3])
-def test_dma_16_byte_runs(gpu, dtype, tile, majors, split):
-    """Real operands whose gathers are 16-byte runs (4 f32 or 2 f64
+def test_dma_16_byte_runs(gpu, dtype, method, tile, majors, split):
+    """Operands whose gathers are 16-byte runs (4 f32, 2 f64 or 2 c32
     consecutive along the major index, every offset a multiple of 16 bytes)
     load whole runs per lane: every tile, both majors of A and B, a batch
-    index, edges that do not fill a tile, K split or not."""
+    index, edges that do not fill a tile, K split or not, and for c32 every
+    complex schedule with conj(A)."""
     M, K, N, H = 148, 340, 136, 2
-    a = T([K, M, H], [1, K, K * M], [2, 0, 3]) if majors[0] == "k" else T([M, K, H], [1, M, M * K], [0, 2, 3])
+    cj = dtype.is_complex
+    a = T([K, M, H], [1, K, K * M], [2, 0, 3], conj=cj) if majors[0] == "k" else \
+        T([M, K, H], [1, M, M * K], [0, 2, 3], conj=cj)
     b = T([K, N, H], [1, K, K * N], [2, 1, 3]) if majors[1] == "k" else T([N, K, H], [1, N, N * K], [1, 2, 3])
     d = T([M, N, H], [1, M, M * N], [0, 1, 3])
     av, bv, cv = values(dtype, a, 61), values(dtype, b, 62), values(dtype, d, 63)
-    got, want, plan = run(dtype, a, av, b, bv, d, np.zeros(d.storage(), NP[dtype]), alpha=0.75, beta=-0.5, c=d,
-                          cv=cv, config=tp.PlanConfig(tile=tile, split_k=split))
+    alpha = complex(0.75, -0.5) if cj else 0.75
+    beta = complex(-0.5, 0.25) if cj else -0.5
+    got, want, plan = run(dtype, a, av, b, bv, d, np.zeros(d.storage(), NP[dtype]), alpha=alpha, beta=beta, c=d,
+                          cv=cv, config=tp.PlanConfig(tile=tile, split_k=split, complex_method=method))
     rep = plan.report()
     assert ".dma16" in rep["kernel"] and rep["a_k_major"] == (majors[0] == "k") and \
         rep["b_k_major"] == (majors[1] == "k"), rep
-    check(dtype, got, want)
+    check(dtype, got, want, method)
 
 
 @pytest.mark.gpu
