@@ -350,9 +350,25 @@ bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double 
   // Every operand the launch touches must be device memory of this device, and
   // every origin an element multiple away from the first item's.
   DeviceGuard dg(ex.device);
-  auto device_origin = [&](const void* q, const void* q0, int64_t& delta) {
+  // Device allocations already seen ([base, end) by base): one pointer query
+  // per allocation instead of one per operand of every item.
+  std::map<uintptr_t, uintptr_t> known;
+  auto on_device = [&](const void* q) {
+    const uintptr_t u = (uintptr_t)q;
+    auto it = known.upper_bound(u);
+    if (it != known.begin() && u < std::prev(it)->second) return true;
     Where w = classify(q, ex.device);
     if (w.host || w.dev != q) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)q) == hipSuccess && size > 0)
+      known[(uintptr_t)base] = (uintptr_t)base + size;
+    else
+      (void)hipGetLastError();
+    return true;
+  };
+  auto device_origin = [&](const void* q, const void* q0, int64_t& delta) {
+    if (!on_device(q)) return false;
     const intptr_t d = (intptr_t)q - (intptr_t)q0;
     if (d % es != 0) return false;
     delta = d / es;

@@ -11,6 +11,7 @@ Operands are torch tensors (device or host), numpy arrays (host, staged by the
 library) or raw integer addresses (`execute_raw`). The library is required:
 nothing here computes a contraction in Python.
 """
+import array
 import ctypes
 import enum
 
@@ -386,7 +387,14 @@ class Plan:
             raise ValueError("a, b, c and d need one address per item")
         pa, _ka = _scalar(alpha, dt)
         pb, _kb = _scalar(beta, dt)
-        arr = lambda xs: (ctypes.c_void_p * max(n, 1))(*[ctypes.c_void_p(x) for x in xs])
+        keep = []
+
+        def arr(xs):
+            # One uint64 buffer viewed as the void* array (no per-address objects).
+            buf = array.array("Q", xs) if n else array.array("Q", [0])
+            keep.append(buf)
+            return (ctypes.c_void_p * len(buf)).from_buffer(buf)
+
         A, B, D = arr(a), arr(b), arr(d)
         C = arr(c) if c is not None else None
         st = _capi.intptr(0)
