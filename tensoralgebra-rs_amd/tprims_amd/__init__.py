@@ -315,7 +315,8 @@ def _addr(x, layout=None):
 def _scalar(v, dtype):
     v = complex(v)
     if dtype == DType.F32:
-        return ctypes.byref(ctypes.c_float(v.real)), ctypes.c_float(v.real)
+        c = ctypes.c_float(v.real)
+        return ctypes.byref(c), c
     if dtype == DType.F64:
         c = ctypes.c_double(v.real)
         return ctypes.byref(c), c
