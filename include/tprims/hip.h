@@ -91,6 +91,35 @@ typedef struct {
 } tprims_plan_report;
 TAPP_EXPORT TAPP_error tprims_product_report(TAPP_tensor_product plan, tprims_plan_report* out);
 
+/* ---- scatter vectors ----------------------------------------------------------
+ * The reference exposes these as Rust API: build_scatter and
+ * build_block_scatter (crates/tprims-kernel/src/pack/scatter.rs:70,130) and
+ * the plan's reduced matrix view (crates/tprims-contract/src/plan/analysis.rs:210,
+ * PackedPlan; pack/scatter.rs:24-31 names them the way for a caller to see the
+ * traversal the engine makes). Each call writes min(length, cap) entries to
+ * out (null allowed when cap is 0) and the full length to *len, so a first call
+ * with cap 0 sizes the buffer. */
+#define TPRIMS_SCATTER_IRREGULAR (-0x7fffffffffffffffLL - 1) /* block not an arithmetic run */
+/* Offsets of the mixed-radix index tuples, fastest mode first; nmode 0 gives
+ * the single offset 0, a zero extent the empty vector. */
+TAPP_EXPORT TAPP_error tprims_build_scatter(const int64_t* extents, const int64_t* strides, int32_t nmode,
+                                            int64_t* out, int64_t cap, int64_t* len);
+/* Per block of blk entries of scat (the last block ragged): the common
+ * difference, 0 for a one-entry block, or TPRIMS_SCATTER_IRREGULAR. blk must
+ * be positive (the reference panics on 0). */
+TAPP_EXPORT TAPP_error tprims_build_block_scatter(const int64_t* scat, int64_t n, int64_t blk, int64_t* out,
+                                                  int64_t cap, int64_t* nblocks);
+/* The plan's scatter vectors in elements, after role assignment, sorting and
+ * folding (the K order is the planner's gather order). An empty role holds the
+ * single offset 0. */
+enum {
+  TPRIMS_SCATTER_AM = 0, TPRIMS_SCATTER_AK, TPRIMS_SCATTER_BK, TPRIMS_SCATTER_BN,
+  TPRIMS_SCATTER_CM, TPRIMS_SCATTER_CN, TPRIMS_SCATTER_DM, TPRIMS_SCATTER_DN,
+  TPRIMS_SCATTER_HA, TPRIMS_SCATTER_HB, TPRIMS_SCATTER_HC, TPRIMS_SCATTER_HD
+};
+TAPP_EXPORT TAPP_error tprims_product_scatter(TAPP_tensor_product plan, int32_t which, int64_t* out, int64_t cap,
+                                              int64_t* len);
+
 /* ---- grouped execution (crates/tprims-contract/src/batch.rs:125) -----------
  * Many products of one dtype (each its own plan and shape) bound to their
  * operands and run as one persistent kernel over a device work list. alpha and

@@ -353,6 +353,60 @@ TAPP_error tprims_product_report(TAPP_tensor_product plan, tprims_plan_report* o
   });
 }
 
+// ---- scatter vectors (crates/tprims-kernel/src/pack/scatter.rs) -------------
+namespace {
+void copy_scatter(const std::vector<int64_t>& v, int64_t* out, int64_t cap, int64_t* len) {
+  if (!len) fail(ERR_INVALID_ARGUMENT, "length pointer is null");
+  if (cap < 0) fail(ERR_INVALID_ARGUMENT, "negative capacity");
+  if (cap > 0 && !out) fail(ERR_INVALID_ARGUMENT, "output buffer is null");
+  const int64_t n = (int64_t)v.size(), k = n < cap ? n : cap;
+  if (k > 0) std::memcpy(out, v.data(), (size_t)k * sizeof(int64_t));
+  *len = n;
+}
+}  // namespace
+
+TAPP_error tprims_build_scatter(const int64_t* extents, const int64_t* strides, int32_t nmode, int64_t* out,
+                                int64_t cap, int64_t* len) {
+  return ffi([&] {
+    if (nmode < 0) fail(ERR_INVALID_ARGUMENT, "negative mode count");
+    if (nmode > 0 && (!extents || !strides)) fail(ERR_INVALID_ARGUMENT, "extents or strides are null");
+    std::vector<int64_t> e(extents, extents + nmode), s(strides, strides + nmode);
+    bool empty = false;
+    for (int64_t x : e) {
+      if (x < 0) fail(ERR_SHAPE, "negative extent");
+      empty = empty || x == 0;
+    }
+    if (!empty) {
+      __int128 total = 1;
+      for (int64_t x : e)
+        if ((total *= x) > ((__int128)1 << 32)) fail(ERR_SHAPE, "scatter vector longer than 2^32 entries");
+    }
+    copy_scatter(empty ? std::vector<int64_t>{} : build_scatter(e, s), out, cap, len);
+  });
+}
+
+TAPP_error tprims_build_block_scatter(const int64_t* scat, int64_t n, int64_t blk, int64_t* out, int64_t cap,
+                                      int64_t* nblocks) {
+  return ffi([&] {
+    if (blk <= 0) fail(ERR_INVALID_ARGUMENT, "block size must be positive");
+    if (n < 0) fail(ERR_INVALID_ARGUMENT, "negative scatter length");
+    if (n > 0 && !scat) fail(ERR_INVALID_ARGUMENT, "scatter vector is null");
+    copy_scatter(build_block_scatter(scat, n, blk), out, cap, nblocks);
+  });
+}
+
+TAPP_error tprims_product_scatter(TAPP_tensor_product plan, int32_t which, int64_t* out, int64_t cap,
+                                  int64_t* len) {
+  return ffi([&] {
+    if (plan == 0) fail(ERR_INVALID_ARGUMENT, "plan is null or zero");
+    const Plan* p = (const Plan*)plan;
+    const std::vector<int64_t>* v[12] = {&p->am, &p->ak, &p->bk, &p->bn, &p->cm, &p->cn,
+                                         &p->dm, &p->dn, &p->ha, &p->hb, &p->hc, &p->hd};
+    if (which < 0 || which >= 12) fail(ERR_INVALID_ARGUMENT, "scatter id out of range");
+    copy_scatter(*v[which], out, cap, len);
+  });
+}
+
 // ---- execution (execute.rs:192-279) -------------------------------------------
 TAPP_error TAPP_execute_product(TAPP_tensor_product plan, TAPP_executor exec, TAPP_status* status, const void* alpha,
                                 const void* A, const void* B, const void* beta, const void* C, void* D) {

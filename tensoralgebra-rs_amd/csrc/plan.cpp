@@ -306,15 +306,16 @@ void configure(Plan* plan, const PlanConfig& cfg) {
     }
   }
   // 16-byte runs (ContractArgs::ab_vec): q = 16 / es consecutive elements
-  // along each operand's major index, every offset a multiple of q.
+  // along each operand's major index, every offset a multiple of q. In block
+  // scatter terms: the major scatter is regular with stride 1 at block size
+  // q, with no ragged last block.
   p.ab_vec = false;
   if (p.off32 && es < 16) {  // f32, f64, c32 (a c64 element is already one 16-byte DMA)
     const int64_t q = 16 / es;
     auto runs = [&](const std::vector<int64_t>& v, int64_t n) {
       if (n % q != 0) return false;
-      for (int64_t i = 0; i < n; i += q)
-        for (int64_t j = 1; j < q; j++)
-          if (v[i + j] != v[i] + j) return false;
+      for (int64_t s : build_block_scatter(v.data(), n, q))
+        if (s != 1) return false;
       return true;
     };
     auto mult = [&](const std::vector<int64_t>& v, int64_t n) {
@@ -338,6 +339,26 @@ void configure(Plan* plan, const PlanConfig& cfg) {
     };
     p.ab_vec = operand(p.a_kmajor, p.am, p.M, p.ak, p.ha) && operand(p.b_kmajor, p.bn, p.N, p.bk, p.hb);
   }
+}
+
+std::vector<int64_t> build_block_scatter(const int64_t* scat, int64_t len, int64_t blk) {
+  std::vector<int64_t> out;
+  if (len <= 0 || blk <= 0) return out;
+  out.reserve((size_t)((len + blk - 1) / blk));
+  for (int64_t lo = 0; lo < len; lo += blk) {
+    const int64_t hi = std::min(lo + blk, len);
+    int64_t stride = 0;
+    if (hi - lo > 1) {
+      stride = scat[lo + 1] - scat[lo];
+      for (int64_t i = lo + 1; i < hi; i++)
+        if (scat[i] - scat[i - 1] != stride) {
+          stride = SCATTER_IRREGULAR;
+          break;
+        }
+    }
+    out.push_back(stride);
+  }
+  return out;
 }
 
 Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {

@@ -86,6 +86,12 @@ void configure(Plan* plan, const PlanConfig& cfg);
 // The scatter vector of a mixed-radix index group, fastest axis first
 // (crates/tprims-kernel/src/pack/scatter.rs:70).
 std::vector<int64_t> build_scatter(const std::vector<int64_t>& extents, const std::vector<int64_t>& strides);
+// The block scatter of scat[0, len) at block size blk > 0: per block of blk
+// consecutive entries (the last one ragged) the common difference of the run,
+// 0 for a run of one entry, or SCATTER_IRREGULAR when the run is not an
+// arithmetic progression (crates/tprims-kernel/src/pack/scatter.rs:130-176).
+constexpr int64_t SCATTER_IRREGULAR = INT64_MIN;
+std::vector<int64_t> build_block_scatter(const int64_t* scat, int64_t len, int64_t blk);
 // Folding of adjacent compatible axes (analysis.rs:299-317).
 std::vector<Axis> fold_axes(const std::vector<Axis>& axes);
 

@@ -20,7 +20,8 @@ from ._capi import TprimsError, check, i64_array, lib
 
 __all__ = [
     "DType", "Op", "LayoutSpec", "OperandSpec", "CSpec", "Labels", "DotGeneral", "Problem", "PlanConfig", "Plan",
-    "Executor", "TprimsError", "ComplexMethod", "Precision", "col_major", "row_major",
+    "Executor", "TprimsError", "ComplexMethod", "Precision", "col_major", "row_major", "build_scatter",
+    "build_block_scatter", "IRREGULAR", "SCATTERS",
 ]
 
 
@@ -312,6 +313,36 @@ def _addr(x, layout=None):
     return x.ctypes.data  # numpy
 
 
+IRREGULAR = -(1 << 63)  # block-scatter entry of a block that is not an arithmetic run (scatter.rs:43)
+# The plan's scatter vectors (the reduced matrix view, analysis.rs:210), by TPRIMS_SCATTER_* id.
+SCATTERS = ("am", "ak", "bk", "bn", "cm", "cn", "dm", "dn", "ha", "hb", "hc", "hd")
+
+
+def _sized(call):
+    """Runs call(out, cap, len_ptr) twice: once to size, once to fill."""
+    n = ctypes.c_int64(0)
+    check(call(None, 0, ctypes.byref(n)))
+    buf = (ctypes.c_int64 * max(1, n.value))()
+    check(call(buf, n.value, ctypes.byref(n)))
+    return list(buf[:n.value])
+
+
+def build_scatter(extents, strides):
+    """Element offsets of the mixed-radix index tuples, fastest mode first
+    (crates/tprims-kernel/src/pack/scatter.rs:70); [] modes give [0]."""
+    if len(extents) != len(strides):
+        raise TprimsError(_capi.TPRIMS_ERR_SHAPE, "extents and strides differ in length")
+    e, s = i64_array(extents), i64_array(strides)
+    return _sized(lambda out, cap, n: lib().tprims_build_scatter(e, s, len(extents), out, cap, n))
+
+
+def build_block_scatter(scat, blk):
+    """Per block of blk entries: the run's common difference, 0 for a single
+    entry, or IRREGULAR (crates/tprims-kernel/src/pack/scatter.rs:130)."""
+    v = i64_array(scat)
+    return _sized(lambda out, cap, n: lib().tprims_build_block_scatter(v, len(scat), int(blk), out, cap, n))
+
+
 def _scalar(v, dtype):
     v = complex(v)
     if dtype == DType.F32:
@@ -359,6 +390,13 @@ class Plan:
     @staticmethod
     def new(problem, config=None):
         return Plan(problem, config)
+
+    def scatters(self):
+        """The plan's twelve scatter vectors (am, ak, bk, bn, cm, cn, dm, dn,
+        ha, hb, hc, hd) after role assignment, sorting and folding
+        (crates/tprims-contract/src/plan/analysis.rs:210, Plan::oriented_scatters)."""
+        return {name: _sized(lambda out, cap, n, i=i: lib().tprims_product_scatter(self.handle, i, out, cap, n))
+                for i, name in enumerate(SCATTERS)}
 
     def report(self):
         """The planner's conclusions (plan/report.rs:33) as a dict."""

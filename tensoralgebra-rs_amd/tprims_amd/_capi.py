@@ -138,6 +138,9 @@ def _declare(lib):
         "tprims_tapp_executor_get_stream": (c_int, [intptr, ctypes.POINTER(vp)]),
         "tprims_tapp_executor_get_device": (c_int, [intptr, ctypes.POINTER(c_int)]),
         "tprims_product_report": (c_int, [intptr, ctypes.POINTER(PlanReportC)]),
+        "tprims_build_scatter": (c_int, [i64p, i64p, ctypes.c_int32, i64p, ctypes.c_int64, i64p]),
+        "tprims_build_block_scatter": (c_int, [i64p, ctypes.c_int64, ctypes.c_int64, i64p, ctypes.c_int64, i64p]),
+        "tprims_product_scatter": (c_int, [intptr, ctypes.c_int32, i64p, ctypes.c_int64, i64p]),
         "tprims_tensor_borrow_raw": (TprimsTensor, [ctypes.POINTER(DLTensor), ctypes.c_uint64]),
         "tprims_execute_product_dlpack": (c_int, [intptr, intptr, vp, TprimsTensor, TprimsTensor, vp, TprimsTensor,
                                                   TprimsTensor]),

@@ -356,3 +356,59 @@ def test_tile_choice_follows_the_busiest_cu_estimate():
     p = make_plan(tp.DType.F64, T([64, 64], [1, 64], [0, 2]), T([64, 64], [1, 64], [2, 1]),
                   T([64, 64], [1, 64], [0, 1]), config=tp.PlanConfig(tile=4)).report()
     assert p["tile"] == 4
+
+
+def test_scatter_vectors_match_the_reference_examples():
+    """build_scatter / build_block_scatter against the reference's own
+    examples (crates/tprims-kernel/src/pack/scatter.rs:62-68, 119-128,
+    138-146, 285-331, 360-366)."""
+    import tprims_amd as tp
+    assert tp.build_scatter([2, 3], [1, 10]) == [0, 1, 10, 11, 20, 21]
+    assert tp.build_scatter([], []) == [0]
+    assert tp.build_scatter([2, 3], [1, 2]) == [0, 1, 2, 3, 4, 5]
+    assert tp.build_scatter([3, 2], [1, 3]) == [0, 1, 2, 3, 4, 5]
+    assert tp.build_scatter([2, 3], [3, 1]) == [0, 3, 1, 4, 2, 5]
+    assert tp.build_scatter([4], [0]) == [0, 0, 0, 0]
+    assert tp.build_scatter([2, 0, 3], [1, 2, 2]) == []
+    s = tp.build_scatter([2, 3], [1, 10])
+    assert tp.build_block_scatter(s, 2) == [1, 1, 1]
+    assert tp.build_block_scatter(s, 3) == [tp.IRREGULAR, tp.IRREGULAR]
+    assert tp.build_block_scatter([0, 1, 2], 2) + tp.build_block_scatter([0, 5], 2) == [1, 0, 5]
+    assert tp.build_block_scatter(tp.build_scatter([2, 3], [3, 1]), 2) == [3, 3, 3]
+    assert tp.build_block_scatter([0, 0, 0, 0], 2) == [0, 0]
+    assert tp.build_block_scatter(list(range(5)), 2) == [1, 1, 0]
+    assert tp.build_block_scatter([], 4) == []
+    with pytest.raises(tp.TprimsError):
+        tp.build_block_scatter([0, 1], 0)  # the reference panics on a zero block size
+    with pytest.raises(tp.TprimsError):
+        tp.build_scatter([-1], [1])
+
+
+def test_plan_scatters_are_the_reduced_matrix_view():
+    """The plan's twelve scatter vectors: a column-major matrix product, and
+    the f64 24^6 contraction (no fold across a and b, K ordered with B's unit
+    stride first), whose M/N runs of 24 are what lets it gather 16-byte runs."""
+    import tprims_amd as tp
+    from harness import T, make_plan
+    p = make_plan(tp.DType.F64, T([4, 6], [1, 4], [0, 2]), T([6, 5], [1, 6], [2, 1]), T([4, 5], [1, 4], [0, 1]))
+    sc = p.scatters()
+    assert sc["am"] == [0, 1, 2, 3] and sc["ak"] == [0, 4, 8, 12, 16, 20]
+    assert sc["bk"] == [0, 1, 2, 3, 4, 5] and sc["bn"] == [0, 6, 12, 18, 24]
+    assert sc["dm"] == sc["cm"] == [0, 1, 2, 3] and sc["dn"] == sc["cn"] == [0, 4, 8, 12, 16]
+    assert sc["ha"] == sc["hb"] == sc["hc"] == sc["hd"] == [0]
+    e = 24
+    st = [1, e, e * e, e ** 3]
+    p = make_plan(tp.DType.F64, T([e] * 4, st, [0, 4, 1, 5]), T([e] * 4, st, [5, 2, 4, 3]), T([e] * 4, st, [0, 1, 2, 3]))
+    sc = p.scatters()
+    assert sc["am"] == tp.build_scatter([e, e], [1, e * e])
+    assert sc["dm"] == tp.build_scatter([e, e], [1, e])
+    assert sc["bn"] == tp.build_scatter([e, e], [e, e ** 3])
+    assert sc["dn"] == tp.build_scatter([e, e], [e * e, e ** 3])
+    assert sc["bk"] == tp.build_scatter([e, e], [1, e * e])
+    assert sc["ak"] == tp.build_scatter([e, e], [e ** 3, e])
+    # runs of 24 along A's M and B's K: regular with stride 1 at the f64 run of 2
+    assert set(tp.build_block_scatter(sc["am"], 2)) == {1} and set(tp.build_block_scatter(sc["bk"], 2)) == {1}
+    assert ".dma16" in p.report()["kernel"]
+    # one odd offset breaks the runs: A with a leading dimension of 25 (odd row starts)
+    q = make_plan(tp.DType.F64, T([6, 8], [1, 7], [2, 0]), T([6, 4], [1, 6], [2, 1]), T([8, 4], [1, 8], [0, 1]))
+    assert q.report()["a_k_major"] == 1 and ".dma16" not in q.report()["kernel"]
