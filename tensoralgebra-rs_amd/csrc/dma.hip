@@ -19,6 +19,11 @@
 //    distinct banks: f32 PADB 4, SW 0; c64 PADB 1, SW 4.
 //  - M-major operands: one row of BM elements per k, lanes along M, pitch
 //    BM + 16 for f32 (lanes of 4 k land on distinct bank quarters).
+//  - 16-byte runs (DCfg::VEC; f32, f64, c32 whose scatters come in runs of
+//    16 / ES consecutive elements): each lane fetches a whole run, K-major
+//    parts as chunk (row, k run) at run * rows + row, M-major parts as chunk
+//    (row run x, k) at k * rows / V + (x ^ XQ * (k % XR)).
+// 8-byte elements outside run layouts arrive as two 4-byte DMAs into planes.
 // Out-of-range rows and k use the buffer sentinels and arrive as zeros.
 // Conjugation is applied to the imaginary parts as they are read for the MFMA
 // (a sign flip of the high word), the f64 MFMA negate flag forms -im*im.
