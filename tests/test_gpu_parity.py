@@ -275,3 +275,56 @@ def test_split_k_repeats_bitwise(gpu, dtype):
     for o in outs:
         np.testing.assert_array_equal(o, got)
 
+
+
+def _random_run_case(rng, dtype):
+    """A random contraction whose A and B gather in 16-byte runs: extents
+    multiples of the run (down to one run), column-major operands with
+    leading dimensions that keep every row start aligned, optional batch."""
+    q = 16 // dtype.size
+    M = int(q * rng.integers(1, 50))
+    N = int(q * rng.integers(1, 40))
+    K = int(q * rng.integers(1, 90))
+    H = int(rng.integers(1, 4))
+    pad = int(q * rng.integers(0, 3))  # extra leading-dimension elements (a multiple of the run)
+    if rng.random() < 0.5:   # A K-major
+        lda = K + pad
+        a = T([K, M, H], [1, lda, lda * M], [2, 0, 3], conj=dtype.is_complex and rng.random() < 0.5)
+    else:
+        lda = M + pad
+        a = T([M, K, H], [1, lda, lda * K], [0, 2, 3], conj=dtype.is_complex and rng.random() < 0.5)
+    if rng.random() < 0.5:   # B K-major
+        ldb = K + pad
+        b = T([K, N, H], [1, ldb, ldb * N], [2, 1, 3])
+    else:
+        ldb = N + pad
+        b = T([N, K, H], [1, ldb, ldb * K], [1, 2, 3])
+    d = T([M, N, H], [1, M, M * N], [0, 1, 3]) if rng.random() < 0.5 else T([M, N, H], [N, 1, M * N], [0, 1, 3])
+    return a, b, d
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.F64, tp.DType.C32], ids=lambda d: d.name)
+def test_dma_16_byte_runs_random_sweep(gpu, dtype):
+    """Seeded random sweep of run layouts (24 cases per dtype): random tile,
+    split and complex schedule, alpha and beta; every case matches the
+    oracle, and most take the run path (the planner may orient an operand
+    whose unit-stride extent is a single short run along its other index)."""
+    rng = np.random.default_rng(1234 + dtype.size)
+    runs = 0
+    methods = [tp.ComplexMethod.NATIVE, tp.ComplexMethod.ONE_M, tp.ComplexMethod.THREE_M]
+    for case in range(24):
+        a, b, d = _random_run_case(rng, dtype)
+        tile = int(rng.choice([1, 3, 4]))
+        split = int(rng.choice([0, 1, 2, 5]))
+        method = methods[int(rng.integers(0, 3))] if dtype.is_complex else tp.ComplexMethod.NATIVE
+        av, bv, cv = values(dtype, a, 300 + case), values(dtype, b, 400 + case), values(dtype, d, 500 + case)
+        alpha = complex(0.5, 0.25) if dtype.is_complex else 1.5
+        beta = complex(-0.25, 0.5) if dtype.is_complex else -0.75
+        got, want, plan = run(dtype, a, av, b, bv, d, np.zeros(d.storage(), NP[dtype]), alpha=alpha, beta=beta, c=d,
+                              cv=cv, config=tp.PlanConfig(tile=tile, split_k=split, complex_method=method))
+        kern = plan.report()["kernel"]
+        runs += ".dma16" in kern
+        print("case", case, kern, a.extents, a.strides, b.extents, b.strides, flush=True)
+        check(dtype, got, want, method)
+    assert runs >= 18, runs
