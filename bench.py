@@ -385,12 +385,16 @@ def copy_calibration(torch):
 
 def cpu_baseline(args):
     """The C++ port of the reference's packed driver on the host cores, timed
-    on one batch entry (n = 1 of 16) of the headline contraction."""
+    on one batch entry (n = 1 of 16) of the headline contraction, on every
+    physical core and (one_thread) on one."""
     try:
         import oracle.packed as packed
     except Exception as e:
         return {"value": None, "unit": "TFLOP/s", "cores": 0, "kind": "port", "sample": "unavailable: %s" % e}
-    return packed.bench_complex_sample(dtype=args.dtype, seconds=10.0)
+    rec = packed.bench_complex_sample(dtype=args.dtype, seconds=10.0)
+    one = packed.bench_complex_sample(dtype=args.dtype, seconds=3.0, threads=1)
+    rec["one_thread"] = {"value": one["value"], "sample": one["sample"]}
+    return rec
 
 
 def config_table(torch, tp, np, ex, stream, steps):
