@@ -412,3 +412,22 @@ def test_plan_scatters_are_the_reduced_matrix_view():
     # one odd offset breaks the runs: A with a leading dimension of 25 (odd row starts)
     q = make_plan(tp.DType.F64, T([6, 8], [1, 7], [2, 0]), T([6, 4], [1, 6], [2, 1]), T([8, 4], [1, 8], [0, 1]))
     assert q.report()["a_k_major"] == 1 and ".dma16" not in q.report()["kernel"]
+
+
+def _build_c_consumer(tmp_path):
+    """Compiles tests/c/consumer.c as C99 against include/ and libtprims."""
+    from tprims_amd import _capi
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.dirname(_capi.LIB_PATH)
+    exe = str(tmp_path / "consumer")
+    subprocess.check_call(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(root, "include"),
+                           os.path.join(root, "tests", "c", "consumer.c"), "-L", libdir, "-ltprims",
+                           "-Wl,-rpath," + libdir, "-o", exe])
+    return exe
+
+
+def test_c_consumer_compiles_against_the_headers(tmp_path):
+    """A C host that uses only the TAPP C ABI and the tprims extensions
+    compiles as strict C99 and links against libtprims (it runs on a GPU:
+    tests/test_gpu_c_consumer.py)."""
+    assert os.path.exists(_build_c_consumer(tmp_path))
