@@ -429,8 +429,9 @@ class Plan:
         keep = []
 
         def arr(xs):
-            # One uint64 buffer viewed as the void* array (no per-address objects).
-            buf = array.array("Q", xs) if n else array.array("Q", [0])
+            # One uint64 buffer viewed as the void* array (no per-address
+            # objects); None is a null pointer, which the library refuses.
+            buf = array.array("Q", [0 if x is None else x for x in xs]) if n else array.array("Q", [0])
             keep.append(buf)
             return (ctypes.c_void_p * len(buf)).from_buffer(buf)
 

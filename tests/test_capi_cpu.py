@@ -431,3 +431,15 @@ def test_c_consumer_compiles_against_the_headers(tmp_path):
     compiles as strict C99 and links against libtprims (it runs on a GPU:
     tests/test_gpu_c_consumer.py)."""
     assert os.path.exists(_build_c_consumer(tmp_path))
+
+
+def test_batched_null_item_pointer_is_refused_before_any_work():
+    """TAPP_execute_batched_product validates every item before running any
+    (crates/tprims-capi/src/execute.rs:235): a null D in the second item is an
+    invalid-argument error, raised without touching a device."""
+    import tprims_amd as tp
+    from harness import T, make_plan
+    p = make_plan(tp.DType.F64, T([4, 3], [1, 4], [0, 2]), T([3, 5], [1, 3], [2, 1]), T([4, 5], [1, 4], [0, 1]))
+    with pytest.raises(tp.TprimsError) as e:
+        p.execute_batched_raw(1.0, [4096, 8192], [4096, 8192], 0.0, None, [8192, None])
+    assert e.value.status == 1
