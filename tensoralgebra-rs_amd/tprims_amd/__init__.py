@@ -431,7 +431,10 @@ class Plan:
         def arr(xs):
             # One uint64 buffer viewed as the void* array (no per-address
             # objects); None is a null pointer, which the library refuses.
-            buf = array.array("Q", [0 if x is None else x for x in xs]) if n else array.array("Q", [0])
+            try:
+                buf = array.array("Q", xs) if n else array.array("Q", [0])
+            except TypeError:
+                buf = array.array("Q", [0 if x is None else x for x in xs])
             keep.append(buf)
             return (ctypes.c_void_p * len(buf)).from_buffer(buf)
 
