@@ -232,14 +232,15 @@ struct ZeroLayout {
 };
 
 // Zero n bytes starting at a 16-byte aligned address (dense layouts). Each
-// workgroup clears one contiguous chunk, four 16-byte stores per lane per
-// step (8 GiB: 5.9 TB/s, against 5.4 for a grid-stride loop).
+// workgroup clears one contiguous chunk, eight 16-byte stores per lane per
+// step (8 GiB: 5.9 TB/s, against 5.4 for a grid-stride loop; the c64 adjoint
+// of the benchmark VJP, 34 GB, 1.5 % faster than with four).
 __global__ void k_zero_bytes(uint4* base, int64_t n16, char* tail, int64_t ntail) {
   const int64_t per = (n16 + gridDim.x - 1) / gridDim.x;
   const int64_t b = (int64_t)blockIdx.x * per, end = b + per < n16 ? b + per : n16;
-  for (int64_t e = b + threadIdx.x; e < end; e += 4 * (int64_t)blockDim.x) {
+  for (int64_t e = b + threadIdx.x; e < end; e += 8 * (int64_t)blockDim.x) {
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < 8; u++)
       if (e + u * blockDim.x < end) base[e + u * blockDim.x] = uint4{0, 0, 0, 0};
   }
   if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
