@@ -34,11 +34,14 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        out = os.path.join(HERE, "_build", "packed_%s.so" % _cpu_tag())
+        src = os.path.join(HERE, "packed_cpu.cpp")
+        with open(src, "rb") as f:
+            tag = hashlib.sha1(f.read()).hexdigest()[:8]
+        out = os.path.join(HERE, "_build", "packed_%s_%s.so" % (_cpu_tag(), tag))
         if not os.path.exists(out):
             os.makedirs(os.path.dirname(out), exist_ok=True)
             subprocess.check_call(["g++", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
-                                   "-o", out + ".tmp", os.path.join(HERE, "packed_cpu.cpp")])
+                                   "-o", out + ".tmp", src])
             os.replace(out + ".tmp", out)
         l = ctypes.CDLL(out)
         i64p = ctypes.POINTER(ctypes.c_int64)
@@ -46,6 +49,9 @@ def lib():
         op = [ctypes.c_void_p, ctypes.c_int, i64p, i64p, i64p, ctypes.c_int]
         l.packed_contract.argtypes = [ctypes.c_int, dp] + op + op + [dp] + op + op + [ctypes.c_int]
         l.packed_contract.restype = ctypes.c_int
+        l.packed_pack_panel.argtypes = [ctypes.c_int, ctypes.c_void_p, i64p, ctypes.c_int64, i64p, ctypes.c_int64,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        l.packed_pack_panel.restype = ctypes.c_int
         _lib = l
     return _lib
 
@@ -79,6 +85,21 @@ def contract(alpha, a, b, beta, c, d, threads=None):
                                int(threads))
     if rc:
         raise ValueError("packed_contract refused the dtype")
+
+
+def pack_panel(data, vscat, kscat, vr, conj=False):
+    """The port's packed panel of `data` (a 1-D array) as reals: Real format
+    for real dtypes, Planar for complex (pack.rs:57)."""
+    data = np.ascontiguousarray(data)
+    dt = DT[data.dtype]
+    nsl = (len(vscat) + vr - 1) // vr
+    real = np.float32 if data.dtype in (np.float32, np.complex64) else np.float64
+    out = np.zeros(nsl * len(kscat) * vr * (2 if np.iscomplexobj(data) else 1), real)
+    rc = lib().packed_pack_panel(dt, data.ctypes.data, _arr(vscat), len(vscat), _arr(kscat), len(kscat), int(vr),
+                                 int(conj), out.ctypes.data)
+    if rc:
+        raise ValueError("packed_pack_panel refused the dtype")
+    return out
 
 
 class _Op:

@@ -85,6 +85,35 @@ struct Desc {
   const int64_t *dims, *strides, *labels;
 };
 
+// One sliver of a packed panel (crates/tprims-kernel/src/pack/pack.rs:57-230,
+// PackFormat::Real and ::Planar): rows v0 .. v0 + vr of the row scatter (zero
+// past vlen) at every k of the K scatter, k-major: real dst[p vr + r]; planar
+// the vr real parts, then the vr imaginary parts of step p at dst[2 p vr].
+// conj negates the imaginary parts.
+template <typename R, bool CPLX>
+void pack_sliver(R* dst, const R* src, int64_t base, const int64_t* vscat, int64_t v0, int64_t vlen, int vr,
+                 const int64_t* kscat, int64_t kc, bool conj) {
+  constexpr int W = CPLX ? 2 : 1;
+  for (int64_t p = 0; p < kc; p++) {
+    R* out = dst + p * W * vr;
+    for (int r = 0; r < vr; r++) {
+      const int64_t v = v0 + r;
+      R re = 0, im = 0;
+      if (v < vlen) {
+        const int64_t off = base + vscat[v] + kscat[p];
+        if (CPLX) {
+          re = src[2 * off];
+          im = conj ? -src[2 * off + 1] : src[2 * off + 1];
+        } else {
+          re = src[off];
+        }
+      }
+      out[r] = re;
+      if (CPLX) out[vr + r] = im;
+    }
+  }
+}
+
 template <typename R, bool CPLX, int MR, int NR>
 void run(const Desc& da, const Desc& db, const Desc& dc, const Desc& dd, const R* A, const R* B, const R* C, R* D,
          const double* alpha, const double* beta, int ca, int cb, int cc, int cd, int threads) {
@@ -158,6 +187,31 @@ void run(const Desc& da, const Desc& db, const Desc& dc, const Desc& dd, const R
     }
   };
 
+  // Empty K: the output pass D = op_D(beta op_C(C)), C never read when beta
+  // is 0 (crates/tprims-kernel/src/pack/writeback.rs:495 scale_only).
+  if (K == 0) {
+    for (int64_t hh = 0; hh < H; hh++)
+      for (int64_t j = 0; j < N; j++)
+        for (int64_t i = 0; i < M; i++) {
+          double vr = 0, vi = 0;
+          if (read_c) {
+            R c_r, c_i;
+            load(C, hc[hh] + cm[i] + cn[j], cc, c_r, c_i);
+            vr = br * c_r - bi * c_i;
+            vi = br * c_i + bi * c_r;
+          }
+          if (cd) vi = -vi;
+          const int64_t doff = hd[hh] + dm[i] + dn[j];
+          if (CPLX) {
+            D[2 * doff] = (R)vr;
+            D[2 * doff + 1] = (R)vi;
+          } else {
+            D[doff] = (R)vr;
+          }
+        }
+    return;
+  }
+
 #pragma omp parallel num_threads(threads)
   for (int64_t hh = 0; hh < H; hh++) {
     for (int64_t jc = 0; jc < N; jc += NC) {
@@ -166,34 +220,18 @@ void run(const Desc& da, const Desc& db, const Desc& dc, const Desc& dd, const R
       for (int64_t pc = 0; pc < K; pc += KC) {
         const int64_t kc = std::min(KC, K - pc);
         const bool first = pc == 0;
-        // Pack B: slivers [sl][p][NR] (planar: re block then im block per sliver).
+        // Pack B: slivers of NR columns.
 #pragma omp for schedule(static)
-        for (int64_t sl = 0; sl < nsl; sl++) {
-          R* dst = bpack.data() + sl * kc * NR * W;
-          for (int64_t p = 0; p < kc; p++)
-            for (int jj = 0; jj < NR; jj++) {
-              int64_t j = jc + sl * NR + jj;
-              R re = 0, im = 0;
-              if (j < N) load(B, hb[hh] + bk[pc + p] + bn[j], cb, re, im);
-              dst[p * NR + jj] = re;
-              if (CPLX) dst[kc * NR + p * NR + jj] = im;
-            }
-        }
+        for (int64_t sl = 0; sl < nsl; sl++)
+          pack_sliver<R, CPLX>(bpack.data() + sl * kc * NR * W, B, hb[hh], bn.data(), jc + sl * NR, N, NR,
+                               bk.data() + pc, kc, cb);
         for (int64_t ic = 0; ic < M; ic += MC) {
           const int64_t mc = std::min(MC, M - ic);
           const int64_t msl = (mc + MR - 1) / MR;
 #pragma omp for schedule(static)
-          for (int64_t sl = 0; sl < msl; sl++) {
-            R* dst = ashared.data() + sl * kc * MR * W;
-            for (int64_t p = 0; p < kc; p++)
-              for (int ii = 0; ii < MR; ii++) {
-                int64_t i = ic + sl * MR + ii;
-                R re = 0, im = 0;
-                if (i < M) load(A, ha[hh] + am[i] + ak[pc + p], ca, re, im);
-                dst[p * MR + ii] = re;
-                if (CPLX) dst[kc * MR + p * MR + ii] = im;
-              }
-          }
+          for (int64_t sl = 0; sl < msl; sl++)
+            pack_sliver<R, CPLX>(ashared.data() + sl * kc * MR * W, A, ha[hh], am.data(), ic + sl * MR, M, MR,
+                                 ak.data() + pc, kc, ca);
 #pragma omp for schedule(static) collapse(2)
           for (int64_t js = 0; js < nsl; js++)
             for (int64_t is = 0; is < msl; is++) {
@@ -203,11 +241,11 @@ void run(const Desc& da, const Desc& db, const Desc& dc, const Desc& dd, const R
               for (int ii = 0; ii < MR; ii++)
                 for (int jj = 0; jj < NR; jj++) cr[ii][jj] = ci[ii][jj] = 0;
               for (int64_t p = 0; p < kc; p++) {
-                const R* a = Ap + p * MR;
-                const R* b = Bp + p * NR;
+                const R* a = Ap + p * MR * W;
+                const R* b = Bp + p * NR * W;
                 if (CPLX) {
-                  const R* aim = Ap + kc * MR + p * MR;
-                  const R* bim = Bp + kc * NR + p * NR;
+                  const R* aim = a + MR;
+                  const R* bim = b + NR;
                   for (int jj = 0; jj < NR; jj++)
                     for (int ii = 0; ii < MR; ii++) {
                       cr[ii][jj] += a[ii] * b[jj] - aim[ii] * bim[jj];
@@ -274,6 +312,28 @@ extern "C" int packed_contract(int dtype, const double* alpha, const void* A, in
     case 3: run<double, true, 8, 4>(xa, xb, xc, xd, (const double*)A, (const double*)B, (const double*)C,
                                     (double*)D, alpha, beta, ca, cb, cc, cd, threads); break;
     default: return 1;
+  }
+  return 0;
+}
+
+// A whole packed panel (ceil(vlen / vr) slivers, pack.rs:57 pack_panel with
+// the Real format for real dtypes and Planar for complex) for tests of the
+// layout; dtype as packed_contract. Returns 1 for an unknown dtype.
+extern "C" int packed_pack_panel(int dtype, const void* src, const int64_t* vscat, int64_t vlen, const int64_t* kscat,
+                                 int64_t kc, int vr, int conj, void* out) {
+  const int64_t nsl = (vlen + vr - 1) / vr;
+  for (int64_t sl = 0; sl < nsl; sl++) {
+    switch (dtype) {
+      case 0: pack_sliver<float, false>((float*)out + sl * kc * vr, (const float*)src, 0, vscat, sl * vr, vlen, vr,
+                                        kscat, kc, conj); break;
+      case 1: pack_sliver<double, false>((double*)out + sl * kc * vr, (const double*)src, 0, vscat, sl * vr, vlen,
+                                         vr, kscat, kc, conj); break;
+      case 2: pack_sliver<float, true>((float*)out + sl * kc * vr * 2, (const float*)src, 0, vscat, sl * vr, vlen,
+                                       vr, kscat, kc, conj); break;
+      case 3: pack_sliver<double, true>((double*)out + sl * kc * vr * 2, (const double*)src, 0, vscat, sl * vr, vlen,
+                                        vr, kscat, kc, conj); break;
+      default: return 1;
+    }
   }
   return 0;
 }

@@ -127,3 +127,45 @@ def test_oracle_plants_are_caught():
     assert good == (1 + 2j) * (2 - 1j) + (3 - 1j) * (1 + 1j)
     flipped = oracle.contract(1.0, a, oracle.Operand(np.conj(b.data), [2], [1], [0]), 0.0, None, d)[0]
     assert flipped != good
+
+
+def test_port_packs_panels_in_the_reference_layout():
+    """The CPU port's packer against the reference's packing examples
+    (crates/tprims-kernel/src/pack/pack.rs:304-330, 331-338, 364-369,
+    393-400, 402-425): Real slivers with zero fill, Planar (the real parts of
+    a k step, then its imaginary parts), conjugation, an irregular row
+    scatter, and the planar size of two k."""
+    from oracle import packed
+    real = packed.pack_panel(np.arange(8, dtype=np.float64), [0, 1, 2], [0, 4], 2)
+    assert real.tolist() == [0.0, 1.0, 4.0, 5.0, 2.0, 0.0, 6.0, 0.0]
+    cplx = np.array([complex(i, 10 + i) for i in range(4)])
+    assert packed.pack_panel(cplx, [0, 1], [0, 2], 2).tolist() == [0.0, 1.0, 10.0, 11.0, 2.0, 3.0, 12.0, 13.0]
+    two = np.array([1 + 2j, 3 + 4j])
+    assert packed.pack_panel(two, [0, 1], [0], 2, conj=True).tolist() == [1.0, 3.0, -2.0, -4.0]
+    assert packed.pack_panel(np.arange(6, dtype=np.float64), [0, 3, 1], [0], 3).tolist() == [0.0, 3.0, 1.0]
+    assert len(packed.pack_panel(two, [0, 1], [0, 0], 2)) == 8
+    assert packed.pack_panel(two.astype(np.complex64), [0, 1], [0], 2).tolist() == [1.0, 3.0, 2.0, 4.0]
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_packed_port_matches_the_oracle(name, dtype):
+    """The CPU baseline (the port of the reference's packed driver) gives the
+    oracle's result on every conformance case and dtype before it is timed:
+    f64 within 1e-11, f32 within 2e-4 (crates/tprims-capi/tests/common/mod.rs:57-63)."""
+    from oracle import packed
+    kw = CASES[name](dtype)
+    a, b, d = kw["a"], kw["b"], kw["d"]
+    c = d if kw.get("in_place") else kw.get("c")
+    cv = kw["dv"] if kw.get("in_place") else kw.get("cv")
+    alpha, beta = kw.get("alpha", 1.0), kw.get("beta", 0.0)
+    if cv is not None and np.any(np.isnan(np.asarray(cv, dtype=complex))):
+        pytest.skip("NaN C is covered by the never-read test")
+    want = oracle.contract(alpha, a.oracle_op(kw["av"]), b.oracle_op(kw["bv"]), beta,
+                           c.oracle_op(cv) if cv is not None else None, d.oracle_op(kw["dv"]), cplx=dtype.is_complex)
+    dv = np.array(kw["dv"], copy=True)
+    dop = d.oracle_op(dv)
+    cop = dop if kw.get("in_place") else (c.oracle_op(np.array(cv, copy=True)) if cv is not None else None)
+    packed.contract(alpha, a.oracle_op(kw["av"]), b.oracle_op(kw["bv"]), beta, cop, dop, threads=2)
+    tol = 2e-4 if dtype.name in ("F32", "C32") else 1e-11
+    assert oracle.max_rel_err(dop.data, want) <= tol, (name, oracle.max_rel_err(dop.data, want))
