@@ -26,7 +26,8 @@ _lib = None
 
 
 def build():
-    """Compile oracle.c (and the packed CPU driver) into oracle/_build."""
+    """Compile oracle.c into oracle/_build (the packed CPU driver, oracle/packed.py,
+    compiles itself on first use for the host CPU)."""
     import subprocess
 
     subprocess.check_call(["make", "-s", "-C", _HERE])
