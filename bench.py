@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-config", default=None,
                     choices=["f64_24", "f64_gemm4096", "transform", "gemm4096", "grouped", "tapp_batched",
-                             "elementwise", "vjp"],
+                             "elementwise", "vjp", "fixed_cost"],
                     help="run one benchmark contraction of the configs table (with --steps and --warmup), print "
                          "its JSON object and exit (for counter passes)")
     return ap.parse_args()
@@ -385,6 +385,26 @@ def copy_calibration(torch):
     print(json.dumps({"copy_calibration_bytes": src.numel() * 8}))
 
 
+def config_fixed_cost():
+    """tests/c/fixed_cost.c: the host cost of one 2x2x2 TAPP_execute_product on
+    device pointers (ns per call; the reference's C-ABI fixed cost is 661 ns on
+    its CPU path, benchmarks/c/README.md:31)."""
+    import subprocess
+    import tempfile
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_capi_cpu import build_fixed_cost
+
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = build_fixed_cost(tmp)
+        out = subprocess.run([exe, "20000"], capture_output=True, text=True, timeout=120)
+    if out.returncode != 0:
+        return {"error": "exit %d: %s" % (out.returncode, out.stderr.strip()[-300:])}
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    rec["workload"] = "2x2x2 f64 TAPP_execute_product on device pointers, 20000 calls per executor mode"
+    return rec
+
+
 def cpu_baseline(args):
     """The C++ port of the reference's packed driver on the host cores, timed
     on one batch entry (n = 1 of 16) of the headline contraction, on every
@@ -408,7 +428,8 @@ def config_table(torch, tp, np, ex, stream, steps):
             ("grouped", lambda: config_grouped(torch, tp, np, ex, stream, steps)),
             ("tapp_batched", lambda: config_tapp_batched(torch, tp, np, ex, stream, steps)),
             ("elementwise", lambda: config_elementwise(torch, tp, np, ex, stream, steps)),
-            ("vjp", lambda: config_vjp(torch, tp, np, ex, stream, steps)))
+            ("vjp", lambda: config_vjp(torch, tp, np, ex, stream, steps)),
+            ("fixed_cost", config_fixed_cost))
 
 
 def main():

@@ -63,13 +63,13 @@ TAPP_EXPORT TAPP_error tprims_create_tensor_product_ex(TAPP_tensor_product* plan
                                                        TAPP_element_op op_D, TAPP_tensor_info D, const int64_t* idx_D,
                                                        TAPP_prectype prec, const tprims_plan_config* cfg);
 
-/* Execute with borrowed DLPack operands (the operand contract of
- * crates/tprims-capi/src/tensor.rs:140-230, extended to ROCm memory). Each
- * view's shape and strides (NULL strides: compact row-major) must equal the
- * extents and strides of the info the plan was built from, its dtype the
- * plan's, lanes 1; devices kDLROCM (the executor's device), kDLROCMHost, kDLCPU
- * or kDLCUDAHost. A D flagged read-only is TPRIMS_ERR_READ_ONLY. C.view may be
- * NULL only with a zero beta. */
+/* Execute with borrowed DLPack operands. Each view passes the operand
+ * contract of tprims_tensor_layout (core.h; crates/tprims-capi/src/tensor.rs:145-230,
+ * extended to ROCm memory: a kDLROCM view must be on the executor's device,
+ * else TPRIMS_ERR_DEVICE), and its shape and strides must equal the extents
+ * and strides of the info the plan was built from (strides of extent-1 modes
+ * are free; else TPRIMS_ERR_SHAPE). A D flagged read-only is
+ * TPRIMS_ERR_READ_ONLY. C.view may be NULL only with a zero beta. */
 TAPP_EXPORT TAPP_error tprims_execute_product_dlpack(TAPP_tensor_product plan, TAPP_executor exec,
                                                      const void* alpha, tprims_tensor A, tprims_tensor B,
                                                      const void* beta, tprims_tensor C, tprims_tensor D);
@@ -124,8 +124,12 @@ TAPP_EXPORT TAPP_error tprims_product_scatter(TAPP_tensor_product plan, int32_t 
  * Many products of one dtype (each its own plan and shape) bound to their
  * operands and run as one persistent kernel over a device work list. alpha and
  * beta arrays hold one scalar per item (of the dtype). All pointers are device
- * memory. The group owns its work list; use_graph captures the launch into a
- * hipGraph that execute then replays. (TAPP_execute_batched_product, for
+ * memory. Items run at the same time, so no item may write bytes another
+ * item reads or writes (span hulls; else TPRIMS_ERR_ALIASED); an item may
+ * read its own output (C is D). The group owns its work list and holds a
+ * reference to each plan (a plan destroyed with TAPP_destroy_tensor_product
+ * stays alive until the group is destroyed); use_graph captures the launch
+ * into a hipGraph that execute then replays. (TAPP_execute_batched_product, for
  * items of one plan, also runs as one launch when every operand is device
  * memory, the precision is the storage one and no item writes bytes another
  * item reads or writes; otherwise its items run in order.) */
@@ -143,7 +147,10 @@ TAPP_EXPORT TAPP_error tprims_group_info(tprims_group group, int64_t* tiles, int
 TAPP_EXPORT TAPP_error tprims_group_set_tile(tprims_group group, int32_t tile_id);
 /* The autotuner: times every storage tile shape on the group's own operands
  * (reps launches each, HIP events on the executor's stream), keeps the fastest
- * and reports it. Operands are overwritten as by tprims_group_execute. */
+ * and reports it. On return every output holds the result of one
+ * tprims_group_execute: outputs that accumulate into themselves (C is D,
+ * beta != 0) are saved before the timing launches and restored before one
+ * final execution on the chosen tile. */
 TAPP_EXPORT TAPP_error tprims_group_autotune(tprims_group group, TAPP_executor exec, int reps, int32_t* best_tile,
                                              double* best_ms);
 TAPP_EXPORT TAPP_error tprims_group_destroy(tprims_group group);

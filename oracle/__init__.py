@@ -11,7 +11,6 @@ Contents
 - `seq()`: the conformance suite's xorshift input sequence
   (crates/tprims-capi/tests/common/mod.rs:72-93), so TAPP cases replay the
   reference suite's exact inputs.
-- `value()`: the deterministic fixture value (fixtures.rs:12-16).
 - `einsum_reference()`: an independent numpy restatement used to cross-check
   `contract()` (numpy.einsum handles diagonals and reductions).
 """
@@ -139,12 +138,6 @@ def seq(n, seed, cplx=False):
         im = nxt() if cplx else 0.0
         out.append(complex(re, im) if cplx else re)
     return np.array(out, dtype=np.complex128 if cplx else np.float64)
-
-
-def value(i, cplx=False):
-    """fixtures::value: re = (i+1)*0.13, im = (i-2)*0.07."""
-    re = (i + 1.0) * 0.13
-    return complex(re, (i - 2.0) * 0.07) if cplx else re
 
 
 def einsum_reference(alpha, a, b, beta, c, d):

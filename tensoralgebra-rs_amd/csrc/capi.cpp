@@ -152,7 +152,8 @@ int execute_items(Plan* p, TAPP_executor exec, const void* alpha, const void* be
   // A batch runs as one launch when that equals running its items in order
   // (TAPP runs them in order, crates/tprims-capi/src/execute.rs:235).
   if (items.size() > 1 && run_items_fused(*p, *ex, al, be, items)) return 0;
-  for (auto& it : items) run_item(*p, *ex, al, be, it);
+  const bool sync = !ex->async.load();
+  for (auto& it : items) run_item(*p, *ex, al, be, it, sync);
   return 0;
 }
 
@@ -342,7 +343,7 @@ TAPP_error tprims_create_tensor_product_ex(TAPP_tensor_product* plan, TAPP_handl
 TAPP_error TAPP_destroy_tensor_product(TAPP_tensor_product plan) {
   return ffi([&] {
     if (plan == 0) fail(ERR_INVALID_ARGUMENT, "plan is null or zero");
-    delete (Plan*)plan;
+    Plan::release((Plan*)plan);  // a group built on it keeps it alive
   });
 }
 
@@ -453,6 +454,11 @@ TAPP_error TAPP_destroy_executor(TAPP_executor exec) {
     if (exec == 0) return;
     Executor* e = (Executor*)exec;
     if (e->inflight.load() > 0) fail(BUSY, "executor has calls in flight");
+    // Work still queued on the executor's stream (an asynchronous call's)
+    // counts as in flight.
+    if (e->busy())
+      fail(BUSY, "work is still queued on the executor's stream; synchronize it first "
+                 "(tprims_tapp_executor_synchronize)");
     delete e;
   });
 }
@@ -534,33 +540,135 @@ TAPP_error tprims_tapp_executor_get_device(TAPP_executor exec, int* device) {
   });
 }
 
-// ---- DLPack execution (tensor.rs:140-230 operand contract) ----------------------
-static const void* dlpack_origin(const tprims_tensor& t, const Operand& o, DType dt, const char* name, bool output) {
-  if (!t.view) fail(ERR_INVALID_ARGUMENT, std::string("null DLTensor for ") + name);
+// ---- DLPack operands (tensor.rs:96-230 operand contract) ----------------------
+namespace {
+
+// A validated borrowed operand (crates/tprims-capi/src/tensor.rs:128-141 Layout).
+struct DlLayout {
+  std::vector<int64_t> dims, strides;
+  const char* origin = nullptr;  // data + byte_offset; a never-read placeholder when empty
+  int64_t lo = 0, hi = 0;        // smallest and largest element offset addressed
+  bool empty = false;
+};
+
+// Address handed out as the origin of an empty tensor: non-null, aligned for
+// every element type, never dereferenced (the reference uses a dangling pointer).
+alignas(16) const char k_empty_origin[16] = {};
+
+DType dl_dtype(const tprims_tensor& t) {  // tensor.rs:96-115
+  if (!t.view) fail(ERR_INVALID_ARGUMENT, "null DLTensor");
+  const DLDataType d = t.view->dtype;
+  if (d.lanes != 1) fail(ERR_DTYPE, "lanes = " + std::to_string(d.lanes) + " (must be 1)");
+  if (d.code == kDLFloat && d.bits == 32) return DType::F32;
+  if (d.code == kDLFloat && d.bits == 64) return DType::F64;
+  if (d.code == kDLComplex && d.bits == 64) return DType::C32;
+  if (d.code == kDLComplex && d.bits == 128) return DType::C64;
+  fail(ERR_DTYPE, "unsupported dtype code " + std::to_string(d.code) + " bits " + std::to_string(d.bits));
+}
+
+// Validates device, dtype, shape, strides and offset with checked arithmetic
+// and a typed status per rejected field, in the order of tensor.rs:145-230;
+// `writable` adds the read-only check of an output (tensor.rs:292-297).
+// ROCm device memory is accepted here; the executions compare its device
+// with the executor's.
+DlLayout dl_layout(const tprims_tensor& t, DType dt, bool writable) {
+  if (!t.view) fail(ERR_INVALID_ARGUMENT, "null DLTensor");
   const DLTensor& v = *t.view;
-  if (v.dtype.lanes != 1) fail(ERR_DTYPE, std::string(name) + ": lanes must be 1");
-  int code = is_complex(dt) ? kDLComplex : kDLFloat;
-  int bits = elem_size(dt) * 8;
-  if (v.dtype.code != code || v.dtype.bits != bits) fail(ERR_DTYPE, std::string(name) + ": dtype differs from the plan's");
   switch (v.device.device_type) {
-    case kDLROCM:
-    case kDLROCMHost:
     case kDLCPU:
-    case kDLCUDAHost: break;
-    default: fail(ERR_DEVICE, std::string(name) + ": unsupported DLPack device type");
+    case kDLCUDAHost:
+    case kDLROCMHost:
+    case kDLROCM: break;
+    default: fail(ERR_DEVICE, "device type " + std::to_string(v.device.device_type) + " is not CPU or ROCm");
   }
-  if (output && (t.flags & DLPACK_FLAG_BITMASK_READ_ONLY)) fail(ERR_READ_ONLY, std::string(name) + " is read-only");
-  if (v.ndim != o.layout.rank()) fail(ERR_SHAPE, std::string(name) + ": rank differs from the plan's");
-  int64_t compact = 1;
-  for (int i = v.ndim - 1; i >= 0; i--) {
-    int64_t st = v.strides ? v.strides[i] : compact;
-    if (v.shape[i] != o.layout.dims[i] || (v.shape[i] > 1 && st != o.layout.strides[i]))
+  if (dl_dtype(t) != dt) fail(ERR_DTYPE, "operand dtypes differ");
+  if (v.ndim < 0) fail(ERR_SHAPE, "negative ndim");
+  const int nd = v.ndim;
+  if (nd > 0 && !v.shape) fail(ERR_INVALID_ARGUMENT, "null shape");
+  DlLayout l;
+  l.dims.assign(nd, 0);
+  l.strides.assign(nd, 0);
+  for (int i = 0; i < nd; i++) {
+    if (v.shape[i] < 0) fail(ERR_SHAPE, "negative extent");
+    l.dims[i] = v.shape[i];
+    l.empty = l.empty || v.shape[i] == 0;
+  }
+  if (!v.strides) {  // compact row-major (the DLPack convention)
+    __int128 acc = 1;
+    for (int i = nd - 1; i >= 0; i--) {
+      l.strides[i] = (int64_t)acc;
+      acc *= l.dims[i] > 1 ? l.dims[i] : 1;
+      if (acc > (__int128)INT64_MAX) fail(ERR_SHAPE, "stride overflow");
+    }
+  } else {
+    for (int i = 0; i < nd; i++) l.strides[i] = v.strides[i];
+  }
+  const int64_t es = elem_size(dt);
+  if (!l.empty) {
+    __int128 lo = 0, hi = 0;
+    for (int i = 0; i < nd; i++) {
+      const __int128 reach = (__int128)(l.dims[i] - 1) * l.strides[i];
+      if (reach < 0) lo += reach;
+      else hi += reach;
+      if (lo < (__int128)INT64_MIN || hi > (__int128)INT64_MAX) fail(ERR_SHAPE, "span overflow");
+    }
+    if (!v.data) fail(ERR_INVALID_ARGUMENT, "null data for a non-empty tensor");
+    if ((hi - lo + 1) * es > (__int128)INT64_MAX) fail(ERR_SHAPE, "span overflow");
+    l.lo = (int64_t)lo;
+    l.hi = (int64_t)hi;
+    if (v.byte_offset > (uint64_t)INT64_MAX) fail(ERR_SHAPE, "byte_offset overflow");
+    l.origin = (const char*)v.data + v.byte_offset;
+    const uintptr_t align = is_double(dt) ? 8 : 4;  // a complex element aligns like its real part
+    if ((uintptr_t)l.origin % align != 0)
+      fail(ERR_INVALID_ARGUMENT, "data + byte_offset is not aligned to " + std::to_string(align) + " bytes");
+  } else {
+    l.origin = k_empty_origin;
+  }
+  if (writable && (t.flags & DLPACK_FLAG_BITMASK_READ_ONLY)) fail(ERR_READ_ONLY, "output tensor is read-only");
+  return l;
+}
+
+// The operand's view must describe the layout the plan was built from.
+const void* dlpack_origin(const tprims_tensor& t, const Operand& o, DType dt, const char* name, bool output) {
+  DlLayout l;
+  try {
+    l = dl_layout(t, dt, output);
+  } catch (const Error& e) {
+    fail(e.status, std::string(name) + ": " + e.what());
+  }
+  if ((int)l.dims.size() != o.layout.rank()) fail(ERR_SHAPE, std::string(name) + ": rank differs from the plan's");
+  for (size_t i = 0; i < l.dims.size(); i++)
+    if (l.dims[i] != o.layout.dims[i] || (l.dims[i] > 1 && l.strides[i] != o.layout.strides[i]))
       fail(ERR_SHAPE, std::string(name) + ": extents or strides differ from the plan's");
-    compact *= v.shape[i];
-  }
-  if (((uintptr_t)v.data + v.byte_offset) % (is_double(dt) ? 8 : 4) != 0)
-    fail(ERR_INVALID_ARGUMENT, std::string(name) + ": misaligned data");
-  return (const char*)v.data + v.byte_offset;
+  return l.origin;
+}
+
+}  // namespace
+
+TAPP_error tprims_tensor_dtype(tprims_tensor t, TAPP_datatype* out) {
+  return ffi([&] {
+    if (!out) fail(ERR_INVALID_ARGUMENT, "dtype out-parameter is null");
+    *out = (TAPP_datatype)dl_dtype(t);
+  });
+}
+
+TAPP_error tprims_tensor_layout(tprims_tensor t, TAPP_datatype dtype, int writable, int64_t* dims, int64_t* strides,
+                                int32_t cap, tprims_tensor_layout_info* out) {
+  return ffi([&] {
+    if (!out) fail(ERR_INVALID_ARGUMENT, "layout out-parameter is null");
+    if (cap < 0 || (cap > 0 && (!dims || !strides))) fail(ERR_INVALID_ARGUMENT, "dims/strides buffers are null");
+    DlLayout l = dl_layout(t, dtype_of(dtype), writable != 0);
+    const int nd = (int)l.dims.size();
+    for (int i = 0; i < nd && i < cap; i++) {
+      dims[i] = l.dims[i];
+      strides[i] = l.strides[i];
+    }
+    out->ndim = nd;
+    out->empty = l.empty;
+    out->origin = (void*)l.origin;
+    out->span_lo = l.lo;
+    out->span_hi = l.hi;
+  });
 }
 
 TAPP_error tprims_execute_product_dlpack(TAPP_tensor_product plan, TAPP_executor exec, const void* alpha,
@@ -574,6 +682,18 @@ TAPP_error tprims_execute_product_dlpack(TAPP_tensor_product plan, TAPP_executor
     const void* b = dlpack_origin(B, p->prob.b, dt, "B", false);
     const void* c = C.view ? dlpack_origin(C, p->prob.c, dt, "C", false) : nullptr;
     void* d = const_cast<void*>(dlpack_origin(D, p->prob.d, dt, "D", true));
+    // Device memory must be the executor's (resolved only now, so that every
+    // field error above is reported without touching the GPU runtime).
+    const tprims_tensor* ops[4] = {&A, &B, &C, &D};
+    const char* names[4] = {"A", "B", "C", "D"};
+    for (int i = 0; i < 4; i++)
+      if (ops[i]->view && ops[i]->view->device.device_type == kDLROCM) {
+        const int device = resolve_executor(exec)->device;
+        if (ops[i]->view->device.device_id != device)
+          fail(ERR_DEVICE, std::string(names[i]) + ": ROCm tensor on device " +
+                               std::to_string(ops[i]->view->device.device_id) + ", the executor on device " +
+                               std::to_string(device));
+      }
     execute_items(p, exec, alpha, beta, {Item{a, b, c, d}});
   });
 }

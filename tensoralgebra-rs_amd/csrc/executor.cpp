@@ -103,6 +103,20 @@ char* Executor::ensure_batch_host(size_t bytes) {
   return batch_host;
 }
 
+void Executor::finish(hipStream_t s, bool sync) {
+  if (sync) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+bool Executor::busy() {
+  std::lock_guard<std::mutex> g(mu);
+  if (!stream_ready || (!stream && !owns_stream)) return false;
+  DeviceGuard dg(device);
+  const hipError_t e = hipStreamQuery(stream);
+  if (e == hipErrorNotReady) return true;
+  if (e != hipSuccess) (void)hipGetLastError();
+  return false;
+}
+
 Executor::~Executor() {
   if (stream_ready && stream) (void)hipStreamSynchronize(stream);
   if (scratch) (void)hipFree(scratch);
@@ -144,13 +158,6 @@ Executor* resolve_executor(intptr_t handle) {
   return e;
 }
 
-namespace {
-
-struct Range {
-  bool present = false;
-  __int128 lo = 0, hi = 0;  // byte addresses [lo, hi)
-};
-
 Range byte_range(const Plan& p, int op, const void* origin) {
   Range r;
   if (!origin) return r;
@@ -171,6 +178,40 @@ Range byte_range(const Plan& p, int op, const void* origin) {
 }
 
 bool overlap(const Range& x, const Range& y) { return x.present && y.present && x.lo < y.hi && y.lo < x.hi; }
+
+bool items_independent(const std::vector<ItemSpans>& items) {
+  const size_t n = items.size();
+  std::vector<std::pair<Range, size_t>> outs;
+  outs.reserve(n);
+  for (size_t i = 0; i < n; i++)
+    if (items[i].d.present) outs.push_back({items[i].d, i});
+  std::sort(outs.begin(), outs.end(), [](const auto& x, const auto& y) { return x.first.lo < y.first.lo; });
+  for (size_t i = 1; i < outs.size(); i++)
+    if (outs[i].first.lo < outs[i - 1].first.hi) return false;
+  // Does r meet an output of an item other than `owner`? Outputs are now
+  // disjoint and sorted, so only those starting below r.hi and ending above
+  // r.lo, found by binary search and walking down, can.
+  auto hits_other_output = [&](const Range& r, size_t owner) {
+    if (!r.present) return false;
+    size_t lo = 0, hi = outs.size();
+    while (lo < hi) {
+      const size_t mid = (lo + hi) / 2;
+      if (outs[mid].first.lo < r.hi) lo = mid + 1;
+      else hi = mid;
+    }
+    for (size_t k = lo; k-- > 0;) {
+      if (outs[k].first.hi <= r.lo) break;
+      if (outs[k].second != owner) return true;
+    }
+    return false;
+  };
+  for (size_t i = 0; i < n; i++)
+    for (const Range& r : items[i].in)
+      if (hits_other_output(r, i)) return false;
+  return true;
+}
+
+namespace {
 
 // Where an operand lives, as seen from the executor's device.
 struct Where {
@@ -258,7 +299,7 @@ void validate_item(const Plan& p, const Item& it, bool beta_zero) {
   }
 }
 
-void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in[2], const Item& it) {
+void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in[2], const Item& it, bool sync) {
   if (p.strategy == 2) return;
   double beta[2] = {beta_in[0], beta_in[1]};
   if (!it.c) beta[0] = beta[1] = 0.0;
@@ -328,8 +369,8 @@ void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta_in
                   "staging copy to the host");
       }
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-  } else if (!ex.async.load()) {
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  } else {
+    ex.finish(s, sync);
   }
 }
 
@@ -388,33 +429,16 @@ bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double 
   // Running the items at once equals running them in order only if no item
   // writes bytes that another item reads or writes (span hulls, conservative).
   if (!p.prob.out_empty) {
-    std::vector<std::pair<Range, size_t>> outs;
-    outs.reserve(n);
-    for (size_t i = 0; i < n; i++) outs.push_back({byte_range(p, OP_D, items[i].d), i});
-    std::sort(outs.begin(), outs.end(), [](const auto& x, const auto& y) { return x.first.lo < y.first.lo; });
-    for (size_t i = 1; i < n; i++)
-      if (outs[i].first.lo < outs[i - 1].first.hi) return false;
-    auto hits_other_output = [&](const Range& r, size_t owner) {
-      if (!r.present) return false;
-      // The last output starting below r.hi; outputs are disjoint and sorted.
-      size_t lo = 0, hi = n;
-      while (lo < hi) {
-        const size_t mid = (lo + hi) / 2;
-        if (outs[mid].first.lo < r.hi) lo = mid + 1;
-        else hi = mid;
-      }
-      for (size_t k = lo; k-- > 0;) {
-        if (outs[k].first.hi <= r.lo) break;
-        if (outs[k].second != owner) return true;
-      }
-      return false;
-    };
+    std::vector<ItemSpans> spans(n);
     for (size_t i = 0; i < n; i++) {
-      if (product && (hits_other_output(byte_range(p, OP_A, items[i].a), i) ||
-                      hits_other_output(byte_range(p, OP_B, items[i].b), i)))
-        return false;
-      if (read_c && hits_other_output(byte_range(p, OP_C, items[i].c), i)) return false;
+      spans[i].d = byte_range(p, OP_D, items[i].d);
+      if (product) {
+        spans[i].in[0] = byte_range(p, OP_A, items[i].a);
+        spans[i].in[1] = byte_range(p, OP_B, items[i].b);
+      }
+      if (read_c) spans[i].in[2] = byte_range(p, OP_C, items[i].c);
     }
+    if (!items_independent(spans)) return false;
   }
 
   // Batch offsets of item i, entry h: the item's origin delta plus the plan's
@@ -461,7 +485,7 @@ bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double 
     if (tiles >= 128) ksplit = 1;
   }
   launch_plan(p, ex, a, product, ksplit, p.kslabs, s);
-  if (!ex.async.load()) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  ex.finish(s, !ex.async.load());
   return true;
 }
 

@@ -47,6 +47,11 @@ struct Executor {
   char* ensure_panels(size_t bytes);   // caller holds panel_mu
   char* ensure_batch(size_t bytes);       // caller holds batch_mu
   char* ensure_batch_host(size_t bytes);  // caller holds batch_mu; waits for the previous upload
+  // Ends a call's stream work: waits for it when sync.
+  void finish(hipStream_t s, bool sync);
+  // Work still queued or running on the executor's stream (an asynchronous
+  // call's, or, on a caller's stream, anyone's): destruction refuses then.
+  bool busy();
   ~Executor();
 };
 
@@ -61,12 +66,33 @@ struct Item {
   void* d;
 };
 
+// Byte addresses [lo, hi) an operand of a plan addresses from its origin;
+// absent for a null origin or an empty layout.
+struct Range {
+  bool present = false;
+  __int128 lo = 0, hi = 0;
+};
+Range byte_range(const Plan& p, int op, const void* origin);
+bool overlap(const Range& x, const Range& y);
+
+// The byte ranges one item of a batch or group writes (d) and reads (in:
+// A and B when the product runs, C when beta != 0).
+struct ItemSpans {
+  Range d;
+  Range in[3];
+};
+// True when running the items at once equals running them in order: no
+// item's output meets another item's output or inputs (span hulls, so
+// conservative). An item may read its own output (C is D).
+bool items_independent(const std::vector<ItemSpans>& items);
+
 // The null rules of the C ABI (crates/tprims-capi/src/execute.rs:30-47) and the
 // plan's alias preflight (crates/tprims-contract/src/plan/mod.rs:384-428).
 void validate_item(const Plan& p, const Item& it, bool beta_zero);
 
-// Runs one validated item (plan/mod.rs:487-530 semantics).
-void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta[2], const Item& it);
+// Runs one validated item (plan/mod.rs:487-530 semantics). sync: wait for
+// the work before returning (staged host operands always wait).
+void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta[2], const Item& it, bool sync);
 
 // Runs validated items of one plan as ONE launch when that gives the same
 // result as running them in order: every operand in device memory, storage

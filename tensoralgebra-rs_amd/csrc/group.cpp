@@ -40,7 +40,7 @@ struct Group {
   int grid = 0;
   int64_t total_tiles = 0;
   int64_t flops = 0;
-  std::vector<Plan*> plans;
+  std::vector<Plan*> plans;          // held (Plan::retain) until the group is destroyed
   std::vector<ContractArgs> args;   // per contraction item, tile fields unset
   std::vector<int> own_tile;        // the tile each item's own plan chose
   std::vector<ContractArgs> passes; // items that only need the output pass
@@ -50,6 +50,10 @@ struct Group {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipStream_t capture = nullptr;  // the legacy null stream cannot be captured
+  // Output byte ranges of the items that read their own output (C is D,
+  // beta != 0): repeated launches would accumulate there, so the autotuner
+  // saves and restores them.
+  std::vector<std::pair<uintptr_t, size_t>> inplace;
 
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -61,6 +65,7 @@ struct Group {
     drop_graph();
     if (capture) (void)hipStreamDestroy(capture);
     if (d_items) (void)hipFree(d_items);
+    for (Plan* p : plans) Plan::release(p);
   }
 };
 
@@ -160,7 +165,7 @@ void run_group(Group& g, Executor& ex) {
   } else {
     launch_all(g, s);
   }
-  if (!ex.async.load()) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  ex.finish(s, !ex.async.load());
 }
 
 // Storage tiles a group can run: 128x64 is not offered for c64, and the
@@ -188,9 +193,8 @@ TAPP_error tprims_group_create(tprims_group* out, TAPP_executor exec, int num_it
     if (num_items > 0 && (!plans || !alpha || !beta || !A || !B || !D))
       fail(ERR_INVALID_ARGUMENT, "a plan, scalar or pointer array is null");
     std::unique_ptr<Group> g(new Group);
-    Executor* ex = resolve_executor(exec);
-    g->ex = ex;
-    // Validate every item before anything is allocated on the device.
+    // Validate every item, and the items against each other, before the GPU
+    // runtime is touched.
     for (int i = 0; i < num_items; i++) {
       if (plans[i] == 0) fail(ERR_INVALID_ARGUMENT, "plan " + std::to_string(i) + " is null");
       Plan* p = (Plan*)plans[i];
@@ -206,15 +210,45 @@ TAPP_error tprims_group_create(tprims_group* out, TAPP_executor exec, int num_it
       read_scalars(beta, g->dtype, i, be);
       const void* c = C ? C[i] : nullptr;
       validate_item(*p, Item{A[i], B[i], c, D[i]}, be[0] == 0 && be[1] == 0);
-      for (const void* q : {A[i], B[i], c, (const void*)D[i]})
-        if (!device_pointer(q, ex->device)) fail(ERR_DEVICE, "grouped operands must be device memory of the executor");
     }
+    // The items run at the same time: none may write bytes another item reads
+    // or writes (the reference's contract_batched gets this from disjoint
+    // mutable borrows, crates/tprims-contract/src/batch.rs:125).
+    std::vector<ItemSpans> spans(num_items);
+    for (int i = 0; i < num_items; i++) {
+      const Plan* p = (const Plan*)plans[i];
+      if (p->prob.out_empty) continue;
+      double al[2], be[2];
+      read_scalars(alpha, g->dtype, i, al);
+      read_scalars(beta, g->dtype, i, be);
+      const void* c = C ? C[i] : nullptr;
+      const bool product = (p->strategy == 0 || p->strategy == 3) && (al[0] != 0 || al[1] != 0);
+      const bool read_c = c && (be[0] != 0 || be[1] != 0);
+      spans[i].d = byte_range(*p, OP_D, D[i]);
+      if (product) {
+        spans[i].in[0] = byte_range(*p, OP_A, A[i]);
+        spans[i].in[1] = byte_range(*p, OP_B, B[i]);
+      }
+      if (read_c) {
+        spans[i].in[2] = byte_range(*p, OP_C, c);
+        if (overlap(spans[i].d, spans[i].in[2]))
+          g->inplace.push_back({(uintptr_t)spans[i].d.lo, (size_t)(spans[i].d.hi - spans[i].d.lo)});
+      }
+    }
+    if (!items_independent(spans))
+      fail(ERR_ALIASED, "grouped items overlap: an item's output meets another item's operands or output");
+    Executor* ex = resolve_executor(exec);
+    g->ex = ex;
+    for (int i = 0; i < num_items; i++)
+      for (const void* q : {A[i], B[i], C ? C[i] : nullptr, (const void*)D[i]})
+        if (!device_pointer(q, ex->device)) fail(ERR_DEVICE, "grouped operands must be device memory of the executor");
     DeviceGuard dg(ex->device);
     std::vector<std::pair<int64_t, int>> order;
     for (int i = 0; i < num_items; i++) order.push_back({((Plan*)plans[i])->K, i});
     std::stable_sort(order.begin(), order.end(), [](auto& x, auto& y) { return x.first > y.first; });
     for (auto& [k, i] : order) {
       Plan* p = (Plan*)plans[i];
+      p->retain();
       g->plans.push_back(p);
       if (p->strategy == 2) continue;
       double al[2], be[2];
@@ -277,7 +311,7 @@ TAPP_error tprims_group_execute_each(tprims_group group, TAPP_executor exec) {
     // Each item on the tile its own plan chose.
     for (size_t i = 0; i < g->args.size(); i++)
       hip_check(launch_contract(g->args[i], g->dtype, g->method, g->own_tile[i], s), "contraction launch");
-    if (!ex->async.load()) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    ex->finish(s, !ex->async.load());
   });
 }
 
@@ -310,6 +344,27 @@ TAPP_error tprims_group_autotune(tprims_group group, TAPP_executor exec, int rep
     hipStream_t s = ex->get_stream();
     const bool graph = g->use_graph;
     g->use_graph = false;
+    // Items that accumulate into their own output are saved first and
+    // restored after the timing launches, so that on return every output
+    // holds the result of exactly one execution.
+    size_t saved_bytes = 0;
+    for (auto& r : g->inplace) saved_bytes += (r.second + 255) & ~(size_t)255;
+    char* saved = nullptr;
+    struct FreeSaved {
+      char*& p;
+      ~FreeSaved() {
+        if (p) (void)hipFree(p);
+      }
+    } free_saved{saved};
+    if (saved_bytes) {
+      hip_check(hipMalloc((void**)&saved, saved_bytes), "hipMalloc(autotune snapshot)");
+      size_t at = 0;
+      for (auto& r : g->inplace) {
+        hip_check(hipMemcpyAsync(saved + at, (const void*)r.first, r.second, hipMemcpyDeviceToDevice, s),
+                  "saving an in-place output");
+        at += (r.second + 255) & ~(size_t)255;
+      }
+    }
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
@@ -337,6 +392,17 @@ TAPP_error tprims_group_autotune(tprims_group group, TAPP_executor exec, int rep
     g->tile = best;
     build_work_list(*g);
     g->use_graph = graph;
+    // One execution on the chosen tile, from the saved in-place outputs.
+    if (saved_bytes) {
+      size_t at = 0;
+      for (auto& r : g->inplace) {
+        hip_check(hipMemcpyAsync((void*)r.first, saved + at, r.second, hipMemcpyDeviceToDevice, s),
+                  "restoring an in-place output");
+        at += (r.second + 255) & ~(size_t)255;
+      }
+    }
+    launch_all(*g, s);
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (best_tile) *best_tile = best;
     if (best_ms) *best_ms = best_t;
   });

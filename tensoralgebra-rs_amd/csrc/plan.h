@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -64,6 +65,14 @@ struct Plan {
   // Device copies, uploaded on first use per device.
   std::mutex mu;
   std::map<int, DeviceScatter> dev;
+
+  // Owners: the TAPP handle and every group built on the plan. The plan is
+  // deleted when the last of them lets go (release).
+  std::atomic<int> refs{1};
+  void retain() { refs.fetch_add(1); }
+  static void release(Plan* p) {
+    if (p && p->refs.fetch_sub(1) == 1) delete p;
+  }
 
   // Adjoint plans (cotangents of A and B), built on the first VJP.
   Plan* adj[2] = {nullptr, nullptr};

@@ -120,12 +120,9 @@ TAPP_error tprims_product_vjp(TAPP_tensor_product plan, TAPP_executor exec, cons
     } rel{ex};
     DeviceGuard dg(ex->device);
     hipStream_t s = ex->get_stream();
-    const int saved_async = ex->async.exchange(1);  // one synchronization at the end
-    struct Restore {
-      Executor* e;
-      int v;
-      ~Restore() { e->async.store(v); }
-    } restore{ex, saved_async};
+    // The adjoint launches are queued back to back; the call waits once at
+    // the end (a synchronous executor) or records its completion.
+    const bool sync = !ex->async.load();
     for (int w = 0; w < 2; w++) {
       Plan* adj = w == 0 ? adjA : adjB;
       if (!adj) continue;
@@ -139,9 +136,9 @@ TAPP_error tprims_product_vjp(TAPP_tensor_product plan, TAPP_executor exec, cons
       }
       const void* x = w == 0 ? dD : A;
       const void* y = w == 0 ? B : dD;
-      run_item(*adj, *ex, alc, accumulate ? one : zero, Item{x, y, accumulate ? out : nullptr, out});
+      run_item(*adj, *ex, alc, accumulate ? one : zero, Item{x, y, accumulate ? out : nullptr, out}, false);
     }
-    if (!saved_async) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    ex->finish(s, sync);
   });
 }
 

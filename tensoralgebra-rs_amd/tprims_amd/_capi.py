@@ -75,6 +75,11 @@ class TprimsTensor(ctypes.Structure):
     _fields_ = [("view", ctypes.POINTER(DLTensor)), ("flags", ctypes.c_uint64)]
 
 
+class TensorLayoutInfoC(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int32), ("empty", ctypes.c_int32), ("origin", vp), ("span_lo", ctypes.c_int64),
+                ("span_hi", ctypes.c_int64)]
+
+
 # Every symbol the public headers declare, in header order (checked by the tests).
 EXPORTED = [
     "TAPP_check_success", "TAPP_explain_error", "TAPP_attr_set", "TAPP_attr_get", "TAPP_attr_clear",
@@ -83,7 +88,8 @@ EXPORTED = [
     "TAPP_set_nmodes", "TAPP_get_extents", "TAPP_set_extents", "TAPP_get_strides", "TAPP_set_strides",
     "TAPP_create_tensor_product", "TAPP_destroy_tensor_product", "TAPP_execute_product",
     "TAPP_execute_batched_product",
-    "tprims_tensor_borrow_versioned", "tprims_tensor_borrow_raw", "tprims_last_error", "tprims_abi_version",
+    "tprims_tensor_borrow_versioned", "tprims_tensor_borrow_raw", "tprims_tensor_dtype", "tprims_tensor_layout",
+    "tprims_last_error", "tprims_abi_version",
     "tprims_has_part", "TAPP_implementation_name", "TAPP_implementation_version",
     "tprims_tapp_executor_create_rayon", "tprims_tapp_executor_set_budget", "tprims_tapp_executor_get_threads",
     "tprims_tapp_executor_create_hip", "tprims_tapp_executor_set_async", "tprims_tapp_executor_synchronize",
@@ -142,6 +148,9 @@ def _declare(lib):
         "tprims_build_block_scatter": (c_int, [i64p, ctypes.c_int64, ctypes.c_int64, i64p, ctypes.c_int64, i64p]),
         "tprims_product_scatter": (c_int, [intptr, ctypes.c_int32, i64p, ctypes.c_int64, i64p]),
         "tprims_tensor_borrow_raw": (TprimsTensor, [ctypes.POINTER(DLTensor), ctypes.c_uint64]),
+        "tprims_tensor_dtype": (c_int, [TprimsTensor, ctypes.POINTER(c_int)]),
+        "tprims_tensor_layout": (c_int, [TprimsTensor, c_int, c_int, i64p, i64p, ctypes.c_int32,
+                                         ctypes.POINTER(TensorLayoutInfoC)]),
         "tprims_execute_product_dlpack": (c_int, [intptr, intptr, vp, TprimsTensor, TprimsTensor, vp, TprimsTensor,
                                                   TprimsTensor]),
         "tprims_group_create": (c_int, [ctypes.POINTER(intptr), intptr, c_int, ctypes.POINTER(intptr), vp,

@@ -4,14 +4,31 @@ contractions (tprims_product_vjp): dA and dB, with diagonal scatter and
 broadcast over reduced indices."""
 import torch
 
-from . import DType, Labels, LayoutSpec, Op, OperandSpec, Plan, PlanConfig, Problem, CSpec
+from . import DType, Executor, Labels, LayoutSpec, Op, OperandSpec, Plan, PlanConfig, Problem, CSpec
+
+_EXECUTORS = {}
+
+
+def _executor(device):
+    """An asynchronous executor on torch's current stream of `device`, so the
+    library's kernels are ordered with the torch work around them (one per
+    stream, kept for the process)."""
+    s = torch.cuda.current_stream(device)
+    if s.cuda_stream == 0:
+        return None  # the null stream: the library's default executor runs there
+    key = (device.index, s.cuda_stream)
+    ex = _EXECUTORS.get(key)
+    if ex is None:
+        ex = _EXECUTORS[key] = Executor(device.index if device.index is not None else 0, s.cuda_stream,
+                                        asynchronous=True)
+    return ex
 
 
 class _Contract(torch.autograd.Function):
     @staticmethod
     def forward(ctx, plan, alpha, a, b, out_shape, out_stride):
         d = torch.empty_strided(out_shape, out_stride, dtype=a.dtype, device=a.device)
-        plan.execute_dlpack(alpha, a, b, 0.0, None, d)
+        plan.execute_dlpack(alpha, a, b, 0.0, None, d, _executor(a.device))
         ctx.plan, ctx.alpha = plan, alpha
         ctx.save_for_backward(a, b)
         ctx.out_stride = out_stride
@@ -32,7 +49,7 @@ class _Contract(torch.autograd.Function):
         if need_b:
             db = torch.empty_strided(b.shape, b.stride(), dtype=b.dtype, device=b.device)
         if need_a or need_b:
-            plan.vjp(ctx.alpha, a, b, g, da, db)
+            plan.vjp(ctx.alpha, a, b, g, da, db, executor=_executor(g.device))
         return None, None, da, db, None, None
 
 

@@ -206,6 +206,27 @@ def large_k(dtype):
     return plain(dtype, a, 201, b, 202, d, alpha=scalar(dtype, 0.75, 0.5))
 
 
+def c_layout_differs(dtype):
+    # tapp_conformance.rs:545-561: C row-major, D column-major over the same indices
+    a, b = T([4, 3], [1, 4], [I, K]), T([3, 5], [1, 3], [K, J])
+    d, c = T([4, 5], [1, 4], [I, J]), T([4, 5], [5, 1], [I, J])
+    return plain(dtype, a, 181, b, 182, d, c=c, cv=values(dtype, c, 183), beta=scalar(dtype, 1.5, -0.5))
+
+
+def empty_k_conj(mask):
+    # tapp_conformance.rs:821-842: k = 0 honours op_C (mask bit 0) and op_D (bit 1)
+    def case(dtype):
+        a, b = T([4, 0], [1, 4], [I, K]), T([0, 5], [1, 0], [K, J])
+        d = T([4, 5], [1, 4], [I, J], conj=bool(mask & 2))
+        c = T([4, 5], [1, 4], [I, J], conj=bool(mask & 1))
+        out = plain(dtype, a, 1, b, 1, d, c=c, cv=values(dtype, d, 271), alpha=scalar(dtype, 2.0, 1.0),
+                    beta=scalar(dtype, -1.5, 0.25))
+        out["av"] = np.zeros(1, dtype=NP[dtype])
+        out["bv"] = np.zeros(1, dtype=NP[dtype])
+        return out
+    return case
+
+
 CASES = {
     "mixed_layout_gemm": mixed_layout_gemm,
     "general_strides": general_strides,
@@ -226,7 +247,10 @@ CASES = {
     "empty_k": empty_k,
     "padded_view": padded_view,
     "large_k": large_k,
+    "c_layout_differs": c_layout_differs,
 }
+for _m in range(4):
+    CASES["empty_k_conj_mask_%d" % _m] = empty_k_conj(_m)
 for _m in range(16):
     CASES["conj_mask_%d" % _m] = conjugation(_m)
     CASES["conj_mask_%d_deep_k" % _m] = conjugation(_m, 600)

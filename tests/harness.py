@@ -75,11 +75,15 @@ def make_plan(dtype, a, b, d, c=None, config=None):
 
 
 def run(dtype, a, av, b, bv, d, dv, alpha=1.0, beta=0.0, c=None, cv=None, in_place=False, config=None,
-        where="gpu", executor=None):
-    """Runs one case; returns (got_storage, want_storage) as numpy arrays."""
+        where="gpu", executor=None, plan=None):
+    """Runs one case (on `plan` when given, else a plan made from the
+    tensors); returns (got_storage, want_storage, plan). With an executor the
+    call runs on its stream (asynchronously when it is set so): the inputs are
+    uploaded and synchronized before, the device synchronized after."""
     import torch
 
-    plan = make_plan(dtype, a, b, d, c, config)
+    if plan is None:
+        plan = make_plan(dtype, a, b, d, c, config)
     es = dtype.size
     npdt = NP[dtype]
     av, bv, dv = (np.asarray(x, dtype=npdt).copy() for x in (av, bv, dv))
@@ -97,7 +101,10 @@ def run(dtype, a, av, b, bv, d, dv, alpha=1.0, beta=0.0, c=None, cv=None, in_pla
         pb = tb.data_ptr() + b.origin() * es
         pd = td.data_ptr() + d.origin() * es
         pc = pd if in_place else (tc.data_ptr() + c.origin() * es if tc is not None else None)
+        torch.cuda.synchronize()  # the uploads, before work on another stream reads them
         plan.execute_raw(alpha, pa, pb, beta, pc, pd, executor)
+        if executor is not None:
+            executor.synchronize()
         torch.cuda.synchronize()
         got = td.cpu().numpy()
     else:  # host buffers, staged through the executor's device scratch

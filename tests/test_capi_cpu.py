@@ -426,6 +426,27 @@ def _build_c_consumer(tmp_path):
     return exe
 
 
+def build_fixed_cost(out_dir):
+    """Compiles tests/c/fixed_cost.c (C99, HIP runtime API for the device
+    buffers) against include/ and libtprims; returns the executable."""
+    from tprims_amd import _capi
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.dirname(_capi.LIB_PATH)
+    exe = os.path.join(str(out_dir), "fixed_cost")
+    subprocess.check_call(["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                           "-I", os.path.join(root, "include"), "-I", "/opt/rocm/include",
+                           os.path.join(root, "tests", "c", "fixed_cost.c"), "-L", libdir, "-ltprims",
+                           "-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath," + libdir, "-Wl,-rpath,/opt/rocm/lib",
+                           "-o", exe])
+    return exe
+
+
+def test_fixed_cost_program_compiles(tmp_path):
+    """The C timing program of one small TAPP_execute_product builds as C99
+    (bench.py runs it on the GPU)."""
+    assert os.path.exists(build_fixed_cost(tmp_path))
+
+
 def test_c_consumer_compiles_against_the_headers(tmp_path):
     """A C host that uses only the TAPP C ABI and the tprims extensions
     compiles as strict C99 and links against libtprims (it runs on a GPU:

@@ -4,7 +4,8 @@ tests/golden/corpus_dot_general.json by tests/golden/make_corpus.py): every
 entry (operand dims and strides, including negative ones, contracted and batch
 axes, conjugation) runs through the library on the GPU with seeded inputs.
 
-Entries the oracle finishes quickly are checked against it in full; larger
+Plans are built by the library's Problem.from_dot_general, as the corpus's
+callers build them. Entries the oracle finishes quickly are checked against it in full; larger
 ones on a fixed sample of output elements, each evaluated in FP64 straight
 from the definition D[out] = sum over the contracted index space of
 op(A) op(B) (crates/tprims-testkit/src/oracle.rs:92-204 restated per element)."""
@@ -74,6 +75,19 @@ def rand(dtype, n, seed):
     return v.astype(NP[dtype])
 
 
+def dot_general_plan(e, dt):
+    """The plan as a caller of the reference's API builds it:
+    Problem.from_dot_general (api/problem.rs:396) over the entry's layouts.
+    The oracle side keeps the test's own label rule (labels() above), so the
+    library's synthetic labels are checked against an independent statement."""
+    conj = e.get("conj", [False, False])
+    spec = lambda k, cj: tp.OperandSpec(tp.LayoutSpec(e[k]["dims"], e[k]["strides"]),
+                                        tp.Op.CONJUGATE if cj else tp.Op.IDENTITY)
+    dot = tp.DotGeneral(e["lc"], e["rc"], e["lb"], e["rb"])
+    prob = tp.Problem.from_dot_general(dt, spec("a", conj[0]), spec("b", conj[1]), spec("c", False), dot)
+    return tp.Plan(prob)
+
+
 def name(e):
     return "%s/%s" % (e["corpus"], e.get("name", "?"))
 
@@ -94,7 +108,7 @@ def test_corpus_entry_vs_oracle(gpu, e):
     a, b, d = tensors(e)
     av, bv = rand(dt, max(1, a.storage()), 11), rand(dt, max(1, b.storage()), 12)
     dv = np.zeros(max(1, d.storage()), NP[dt])
-    got, want, plan = run(dt, a, av, b, bv, d, dv)
+    got, want, plan = run(dt, a, av, b, bv, d, dv, plan=dot_general_plan(e, dt))
     check(dt, got, want)
 
 
@@ -105,8 +119,7 @@ def test_corpus_entry_sampled(gpu, e):
 
     dt = DT[e["dtype"]]
     a, b, d = tensors(e)
-    from harness import make_plan
-    plan = make_plan(dt, a, b, d)
+    plan = dot_general_plan(e, dt)
     dev = torch.device("cuda:0")
     tdt = {tp.DType.F32: torch.float32, tp.DType.F64: torch.float64, tp.DType.C32: torch.complex64,
            tp.DType.C64: torch.complex128}[dt]

@@ -163,3 +163,81 @@ def test_group_rejects_mixed_dtypes_without_a_gpu_call():
         tp.Group([p1, p2], [1.0, 1.0], [bufs[0], bufs[0]], [bufs[1], bufs[1]], [0.0, 0.0], None,
                  [bufs[2], bufs[3]])
     assert e.value.status in (3, 4)
+
+
+def _gemm_plan(dtype=tp.DType.F64):
+    return make_plan(dtype, T([4, 3], [1, 4], [0, 2]), T([3, 5], [1, 3], [2, 1]), T([4, 5], [1, 4], [0, 1]))
+
+
+def test_group_rejects_overlapping_items_without_a_gpu_call():
+    """Items run at the same time: an item whose output meets another item's
+    operand or output is refused (TPRIMS_ERR_ALIASED) before the GPU runtime
+    is touched; disjoint items and an item reading its own output pass that
+    check (and then need a device, absent here)."""
+    p = _gemm_plan()
+    buf = np.zeros(256)
+    base = buf.ctypes.data
+    A0, B0, D0 = base, base + 8 * 16, base + 8 * 40
+    A1, B1, D1 = base + 8 * 80, base + 8 * 100, base + 8 * 130
+    cases = {
+        "D1 on A0": ([A0, A1], [B0, B1], [D0, A0 + 8], None, 0.0),
+        "D1 on D0": ([A0, A1], [B0, B1], [D0, D0 + 8 * 5], None, 0.0),
+        "D0 on C1": ([A0, A1], [B0, B1], [D0, D1], [D1, D0], 1.0),
+    }
+    for label, (A, B, D, C, beta) in cases.items():
+        with pytest.raises(tp.TprimsError) as e:
+            tp.Group([p, p], [1.0, 1.0], A, B, [beta, beta], C, D)
+        assert e.value.status == 6, label  # TPRIMS_ERR_ALIASED
+    # disjoint items, the second accumulating into itself: past the check
+    for C, betas in ((None, [0.0, 0.0]), ([D0, D1], [0.0, 1.0])):
+        try:
+            tp.Group([p, p], [1.0, 1.0], [A0, A1], [B0, B1], betas, C, [D0, D1])
+        except tp.TprimsError as e:
+            assert e.status == 4  # no HIP device in this container
+        else:  # a GPU machine: host buffers are not grouped operands
+            pytest.fail("host operands were accepted")
+
+
+@pytest.mark.gpu
+def test_group_autotune_with_in_place_items_runs_them_once(gpu):
+    """Items with C = D and beta = 1 accumulate into their output: after
+    autotune (many timing launches) every output holds one execution's
+    result, D0 + alpha * L M, checked against the oracle."""
+    import torch
+
+    dtype = tp.DType.C64
+    items = mps_items(dtype, 12, 13)
+    plans, al, A, B, D, want = [], [], [], [], [], []
+    for k, (L, lv, Mt, mv, E, cv) in enumerate(items):
+        plans.append(make_plan(dtype, L, Mt, E))
+        A.append(torch.from_numpy(lv).to(gpu))
+        B.append(torch.from_numpy(mv).to(gpu))
+        D.append(torch.from_numpy(cv.copy()).to(gpu))
+        a = complex(0.5 + 0.1 * k, 0.3)
+        al.append(a)
+        want.append(oracle.contract(a, L.oracle_op(lv), Mt.oracle_op(mv), 1.0, E.oracle_op(cv),
+                                    E.oracle_op(np.zeros(len(cv), NP[dtype])), cplx=True))
+    g = tp.Group(plans, al, A, B, [1.0] * len(plans), D, D)
+    tile, ms = g.autotune(reps=3)
+    torch.cuda.synchronize()
+    check_all(dtype, D, want, "after autotune (tile %d)" % tile)
+
+
+@pytest.mark.gpu
+def test_group_keeps_its_plans_alive(gpu):
+    """Plans destroyed after the group was built stay valid for it (the
+    group holds a reference to each)."""
+    import torch
+
+    dtype = tp.DType.F64
+    plans, al, be, A, B, C, D, want = build(dtype, mps_items(dtype, 8, 17), gpu, beta=0.5)
+    g = tp.Group(plans, al, A, B, be, C, D)
+    for p in plans:
+        p.close()
+    plans.clear()
+    import gc
+    gc.collect()
+    g.execute()
+    torch.cuda.synchronize()
+    check_all(dtype, D, want, "plans released")
+    g.close()

@@ -143,3 +143,45 @@ def test_autograd_function_gradcheck(gpu, dtype):
     # forward agrees with torch's own einsum
     ref = 0.5 * torch.einsum("iijrb,jkb->ikb", A.detach().cpu(), B.detach().cpu())
     assert torch.allclose(f(A, B).detach().cpu(), ref, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["float32", "complex128"])
+def test_autograd_on_a_side_stream(gpu, dtype):
+    """Forward and backward under torch.cuda.stream(s): the library's kernels
+    run on s, ordered with the torch work that produces their inputs and
+    consumes their outputs on s (no host synchronization in between)."""
+    import torch
+
+    from tprims_amd.autograd import contract
+
+    dt = getattr(torch, dtype)
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        g = torch.Generator(device=gpu).manual_seed(2)
+        A0 = torch.randn(64, 64, 48, 3, 4, dtype=dt, device=gpu, generator=g)
+        B0 = torch.randn(48, 56, 4, dtype=dt, device=gpu, generator=g)
+        A = (A0 * 1.5).requires_grad_()  # produced on s just before the forward reads it
+        B = (B0 - 0.25).requires_grad_()
+        D = contract("iijrb", A, "jkb", B, "ikb", alpha=0.5)
+        W = torch.randn(D.shape, dtype=dt, device=gpu, generator=g)
+        (D * W.conj()).real.sum().backward()
+        out = (D.detach().clone(), A.grad.clone(), B.grad.clone())
+    s.synchronize()
+    a64 = A.detach().cpu().to(torch.complex128 if A.is_complex() else torch.float64)
+    b64 = B.detach().cpu().to(a64.dtype)
+    w64 = W.cpu().to(a64.dtype)
+    ref = 0.5 * torch.einsum("iijrb,jkb->ikb", a64, b64)
+    tol_ = 1e-4 if dtype == "float32" else 1e-12
+    scale = ref.abs().max()
+    assert (out[0].cpu().to(a64.dtype) - ref).abs().max() <= tol_ * scale
+    # dB = conj(alpha) sum_{i,r} conj(A[i,i,j,r,b]) W[i,k,b]
+    diag = torch.diagonal(a64, dim1=0, dim2=1).movedim(-1, 0)  # [i, j, r, b]
+    want_db = 0.5 * torch.einsum("ijrb,ikb->jkb", diag.conj(), w64)
+    assert (out[2].cpu().to(a64.dtype) - want_db).abs().max() <= tol_ * want_db.abs().max()
+    want_da_diag = 0.5 * torch.einsum("ikb,jkb->ijb", w64, b64.conj())
+    got_da = out[1].cpu().to(a64.dtype)
+    got_diag = torch.diagonal(got_da, dim1=0, dim2=1).movedim(-1, 0)  # [i, j, r, b]
+    for r in range(got_diag.shape[2]):
+        assert (got_diag[:, :, r, :] - want_da_diag).abs().max() <= tol_ * want_da_diag.abs().max()

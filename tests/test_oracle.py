@@ -75,6 +75,68 @@ def test_seq_is_the_reference_sequence_class():
     np.testing.assert_array_equal(z.imag, oracle.seq(20, 11)[1::2])
 
 
+def _gf2_apply(cols, v):
+    """A 64x64 matrix over GF(2), given by the images of the 64 unit vectors,
+    applied to the 64-bit vector v."""
+    out = 0
+    j = 0
+    while v:
+        if v & 1:
+            out ^= cols[j]
+        v >>= 1
+        j += 1
+    return out
+
+
+def _gf2_pow(cols, n):
+    result = [1 << j for j in range(64)]
+    base = list(cols)
+    while n:
+        if n & 1:
+            result = [_gf2_apply(base, c) for c in result]
+        base = [_gf2_apply(base, c) for c in base]
+        n >>= 1
+    return result
+
+
+def test_seq_generator_has_the_full_period_and_the_published_constants():
+    """seq's state update (crates/tprims-capi/tests/common/mod.rs:86-91) is a
+    linear map T over GF(2)^64. Its shift triple is pinned by the period: T
+    has order exactly 2^64 - 1 (T^(2^64-1) = I and T^((2^64-1)/p) != I for
+    every prime p dividing 2^64 - 1), which holds for the published triple
+    (13, 7, 17) and fails for a wrong shift or order. The state is one
+    xorshift step per value drawn from the scrambled seed; the scrambling
+    constant is floor(2^64 / golden ratio), odd."""
+    mask = (1 << 64) - 1
+    # T's columns: run one step of the generator's update on each unit vector.
+    def step(s):
+        s ^= (s << 13) & mask
+        s ^= s >> 7
+        s ^= (s << 17) & mask
+        return s
+    cols = [step(1 << j) for j in range(64)]
+    # The step actually used by oracle.seq: its first value from a seed equals
+    # the map applied to the scrambled seed.
+    state = ((5 * 0x9E3779B97F4A7C15) & mask) | 1
+    assert oracle.seq(1, 5)[0] == (_gf2_apply(cols, state) >> 52) / 2048.0 - 1.0
+    n = mask
+    ident = [1 << j for j in range(64)]
+    assert _gf2_pow(cols, n) == ident
+    for p in (3, 5, 17, 257, 641, 65537, 6700417):
+        assert n % p == 0
+        assert _gf2_pow(cols, n // p) != ident, p
+    import math
+    assert 3 * 5 * 17 * 257 * 641 * 65537 * 6700417 == n
+    golden = (math.isqrt(5 << 128) - (1 << 64)) // 2  # floor(2^64 (sqrt(5) - 1) / 2)
+    assert golden == 0x9E3779B97F4A7C15 and golden & 1
+    # Values: the state's top 12 bits scaled into [-1, 1), every multiple of
+    # 1/2048 reached over one long run.
+    x = oracle.seq(200000, 3)
+    assert x.min() >= -1.0 and x.max() < 1.0
+    assert np.all(x * 2048 == np.round(x * 2048))
+    assert len(np.unique(x)) == 4096
+
+
 def test_max_rel_err_definition():
     # fixtures.rs:81-87: normalised by max(1, max |want|)
     assert oracle.max_rel_err(np.array([1.5, 0.0]), np.array([1.0, 0.0])) == 0.5
