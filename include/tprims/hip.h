@@ -74,6 +74,26 @@ TAPP_EXPORT TAPP_error tprims_execute_product_dlpack(TAPP_tensor_product plan, T
                                                      const void* alpha, tprims_tensor A, tprims_tensor B,
                                                      const void* beta, tprims_tensor C, tprims_tensor D);
 
+/* ---- execution without allocation -------------------------------------------
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and results as
+ * TAPP_execute_product, and it calls no hipMalloc, hipFree or allocating
+ * library routine: the plan's scatter vectors and the split-K partial tiles
+ * go to one static device array of TPRIMS_NOALLOC_SCRATCH_BYTES per GPU
+ * (held from the library's first use of a GPU until the process ends), the
+ * scatter vectors copied there on the executor's stream by every call.
+ * Limits: operands in device memory of the executor's device (else
+ * TPRIMS_ERR_DEVICE), the storage precision (else TPRIMS_ERR_UNSUPPORTED), and
+ * scatter vectors plus split-K partials within the array (else
+ * TPRIMS_ERR_UNSUPPORTED; a plan with split_k = 1 needs no partials). All
+ * calls on one GPU share its array: calls that overlap in time on one GPU
+ * (different streams or threads) race; calls on one stream are ordered. */
+#define TPRIMS_NOALLOC_SCRATCH_BYTES (64u << 20)
+TAPP_EXPORT TAPP_error tprims_execute_product_noalloc(TAPP_tensor_product plan, TAPP_executor exec,
+                                                      TAPP_status* status, const void* alpha, const void* A,
+                                                      const void* B, const void* beta, const void* C, void* D);
+
 /* What the planner concluded (crates/tprims-contract/src/plan/report.rs:33). */
 typedef struct {
   int64_t m, n, k, batch;   /* reduced matrix view after folding */

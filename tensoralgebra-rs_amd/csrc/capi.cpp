@@ -30,6 +30,29 @@ void clear_last_error() {
   if (!g_last_error.empty()) g_last_error.clear();
 }
 
+// Reads alpha/beta as the plan's element type.
+void read_scalar(const void* p, DType t, const char* name, double out[2]) {
+  if (!p) fail(ERR_INVALID_ARGUMENT, std::string(name) + " is null or zero");
+  out[0] = out[1] = 0.0;
+  switch (t) {
+    case DType::F32: {
+      float v;
+      memcpy(&v, p, 4);
+      out[0] = v;
+      break;
+    }
+    case DType::F64: memcpy(out, p, 8); break;
+    case DType::C32: {
+      float v[2];
+      memcpy(v, p, 8);
+      out[0] = v[0];
+      out[1] = v[1];
+      break;
+    }
+    case DType::C64: memcpy(out, p, 16); break;
+  }
+}
+
 }  // namespace tp
 
 namespace {
@@ -73,29 +96,6 @@ std::vector<int64_t> labels_of(const int64_t* p, const TensorInfo& t, const char
   if (n == 0) return {};
   if (!p) fail(ERR_INVALID_ARGUMENT, std::string("labels of ") + name + " is null or zero");
   return std::vector<int64_t>(p, p + n);
-}
-
-// Reads alpha/beta as the plan's element type.
-void read_scalar(const void* p, DType t, const char* name, double out[2]) {
-  if (!p) fail(ERR_INVALID_ARGUMENT, std::string(name) + " is null or zero");
-  out[0] = out[1] = 0.0;
-  switch (t) {
-    case DType::F32: {
-      float v;
-      memcpy(&v, p, 4);
-      out[0] = v;
-      break;
-    }
-    case DType::F64: memcpy(out, p, 8); break;
-    case DType::C32: {
-      float v[2];
-      memcpy(v, p, 8);
-      out[0] = v[0];
-      out[1] = v[1];
-      break;
-    }
-    case DType::C64: memcpy(out, p, 16); break;
-  }
 }
 
 int create_product(TAPP_tensor_product* plan_out, TAPP_handle handle, TAPP_element_op op_a, TAPP_tensor_info a,

@@ -12,6 +12,9 @@ namespace tp {
 
 void set_last_error(const std::string& m);
 void clear_last_error();
+// Reads alpha or beta (`name`) as the plan's element type into (re, im);
+// null is TPRIMS_ERR_INVALID_ARGUMENT.
+void read_scalar(const void* p, DType t, const char* name, double out[2]);
 
 template <class F>
 int ffi(F&& body) {

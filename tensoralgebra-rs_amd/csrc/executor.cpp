@@ -211,14 +211,6 @@ bool items_independent(const std::vector<ItemSpans>& items) {
   return true;
 }
 
-namespace {
-
-// Where an operand lives, as seen from the executor's device.
-struct Where {
-  const void* dev = nullptr;  // usable device address of the origin
-  bool host = false;          // pageable host memory: staged
-};
-
 Where classify(const void* p, int device) {
   Where w;
   if (!p) return w;
@@ -248,13 +240,14 @@ Where classify(const void* p, int device) {
   }
 }
 
+namespace {
+
 bool is_zero(const double v[2]) { return v[0] == 0.0 && v[1] == 0.0; }
 
-// The launches of one planned call on device-resident arguments: output pass,
-// elementwise strategy, FP8, split-BF16, split K (partials then the ordered
-// reduction) or the contraction kernel. ksplit <= 1 runs K in one range.
-void launch_plan(Plan& p, Executor& ex, ContractArgs& a, bool product, int ksplit, int64_t kslabs, hipStream_t s) {
-  const int64_t es = elem_size(p.prob.dtype);
+}  // namespace
+
+void launch_plan(Plan& p, Executor& ex, ContractArgs& a, bool product, int ksplit, int64_t kslabs, hipStream_t s,
+                 void* work) {
   if (!product) {
     hip_check(launch_output_pass(a, p.prob.dtype, s), "output-pass launch");
   } else if (p.strategy == 3) {
@@ -269,10 +262,15 @@ void launch_plan(Plan& p, Executor& ex, ContractArgs& a, bool product, int kspli
     char* panels = ex.ensure_panels(split_panel_bytes(a, p.prob.dtype, terms, p.tile));
     hip_check(launch_contract_split(a, p.prob.dtype, terms, p.tile, panels, s), "split-BF16 contraction launch");
   } else if (ksplit > 1) {
-    std::lock_guard<std::mutex> g(ex.panel_mu);
+    std::unique_lock<std::mutex> g(ex.panel_mu, std::defer_lock);
     a.ksplit = ksplit;
     a.kslabs = kslabs;
-    a.work = ex.ensure_panels((size_t)ksplit * (size_t)(a.M * a.N * a.H) * (size_t)es);
+    if (work) {
+      a.work = work;
+    } else {
+      g.lock();
+      a.work = ex.ensure_panels(splitk_work_bytes(p, ksplit, a.H));
+    }
     hip_check(launch_contract(a, p.prob.dtype, p.cfg.complex_method, p.tile, s), "contraction launch");
     hip_check(launch_splitk_reduce(a, p.prob.dtype, s), "split-K reduction launch");
   } else {
@@ -280,7 +278,9 @@ void launch_plan(Plan& p, Executor& ex, ContractArgs& a, bool product, int kspli
   }
 }
 
-}  // namespace
+size_t splitk_work_bytes(const Plan& p, int ksplit, int64_t H) {
+  return (size_t)ksplit * (size_t)(p.M * p.N * H) * (size_t)elem_size(p.prob.dtype);
+}
 
 void validate_item(const Plan& p, const Item& it, bool beta_zero) {
   if (!it.a || !it.b || !it.d) fail(ERR_INVALID_ARGUMENT, "A, B or D data pointer is null or zero");

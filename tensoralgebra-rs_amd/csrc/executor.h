@@ -86,6 +86,16 @@ struct ItemSpans {
 // conservative). An item may read its own output (C is D).
 bool items_independent(const std::vector<ItemSpans>& items);
 
+// Where an operand lives, as seen from an executor's device: device (or
+// managed, or pinned host) memory usable in place at `dev`, or pageable host
+// memory (`host`), which run_item stages. Device memory of another device is
+// TPRIMS_ERR_DEVICE.
+struct Where {
+  const void* dev = nullptr;
+  bool host = false;
+};
+Where classify(const void* p, int device);
+
 // The null rules of the C ABI (crates/tprims-capi/src/execute.rs:30-47) and the
 // plan's alias preflight (crates/tprims-contract/src/plan/mod.rs:384-428).
 void validate_item(const Plan& p, const Item& it, bool beta_zero);
@@ -102,6 +112,15 @@ void run_item(Plan& p, Executor& ex, const double alpha[2], const double beta[2]
 // items must run one by one.
 bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double beta[2],
                      const std::vector<Item>& items);
+
+// The launches of one planned call on device-resident arguments: output pass,
+// elementwise strategy, FP8, split-BF16, split K (partials then the ordered
+// reduction) or the contraction kernel. ksplit <= 1 runs K in one range.
+// Split K writes its partial tiles to `work` (splitk_work_bytes) when given,
+// else to the executor's panel arena (grown as needed).
+void launch_plan(Plan& p, Executor& ex, ContractArgs& a, bool product, int ksplit, int64_t kslabs, hipStream_t s,
+                 void* work = nullptr);
+size_t splitk_work_bytes(const Plan& p, int ksplit, int64_t H);
 
 void hip_check(hipError_t e, const char* what);
 

@@ -392,11 +392,9 @@ Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
   return p.release();
 }
 
-const DeviceScatter& Plan::on_device(int device) {
-  std::lock_guard<std::mutex> g(mu);
-  auto it = dev.find(device);
-  if (it != dev.end()) return it->second;
-  std::vector<const std::vector<int64_t>*> parts = {&am, &ak, &bk, &bn, &cm, &cn, &dm, &dn, &ha, &hb, &hc, &hd, &dr};
+std::vector<int64_t> Plan::flat_tables() const {
+  const std::vector<const std::vector<int64_t>*> parts = {&am, &ak, &bk, &bn, &cm, &cn, &dm,
+                                                          &dn, &ha, &hb, &hc, &hd, &dr};
   size_t total = 0;
   for (auto* v : parts) total += v->size();
   // The 32-bit K offsets follow, two per 64-bit word (each array starts on a word).
@@ -407,33 +405,60 @@ const DeviceScatter& Plan::on_device(int device) {
   flat.resize(total + w32a + w32b, 0);
   if (!ak32.empty()) std::memcpy(flat.data() + total, ak32.data(), ak32.size() * 4);
   if (!bk32.empty()) std::memcpy(flat.data() + total + w32a, bk32.data(), bk32.size() * 4);
-  const size_t words = flat.size();
-  int prev = 0;
-  (void)hipGetDevice(&prev);
-  if (prev != device) (void)hipSetDevice(device);
+  return flat;
+}
+
+DeviceScatter Plan::tables_at(int64_t* base) const {
   DeviceScatter ds{};
-  hipError_t e = hipMalloc(&ds.base, words * sizeof(int64_t));
-  if (e == hipSuccess) e = hipMemcpy(ds.base, flat.data(), words * sizeof(int64_t), hipMemcpyHostToDevice);
-  if (prev != device) (void)hipSetDevice(prev);
-  if (e != hipSuccess) {
-    if (ds.base) (void)hipFree(ds.base);
-    fail(ERR_INTERNAL, std::string("uploading scatter vectors: ") + hipGetErrorString(e));
-  }
-  const int64_t* q = ds.base;
+  ds.base = base;
+  const std::vector<const std::vector<int64_t>*> parts = {&am, &ak, &bk, &bn, &cm, &cn, &dm,
+                                                          &dn, &ha, &hb, &hc, &hd, &dr};
+  const int64_t* q = base;
   const int64_t** slots[] = {&ds.am, &ds.ak, &ds.bk, &ds.bn, &ds.cm, &ds.cn, &ds.dm,
                              &ds.dn, &ds.ha, &ds.hb, &ds.hc, &ds.hd, &ds.dr};
   for (size_t i = 0; i < parts.size(); i++) {
     *slots[i] = q;
     q += parts[i]->size();
   }
+  const size_t w32a = (ak32.size() + 1) / 2;
   ds.ak32 = ak32.empty() ? nullptr : (const uint32_t*)q;
   ds.bk32 = bk32.empty() ? nullptr : (const uint32_t*)(q + w32a);
-  return dev.emplace(device, ds).first->second;
+  return ds;
+}
+
+const std::vector<int64_t>& Plan::host_tables() {
+  std::lock_guard<std::mutex> g(mu);
+  if (host_image.empty()) host_image = flat_tables();
+  return host_image;
+}
+
+const DeviceScatter& Plan::on_device(int device) {
+  std::lock_guard<std::mutex> g(mu);
+  auto it = dev.find(device);
+  if (it != dev.end()) return it->second;
+  const std::vector<int64_t> flat = flat_tables();
+  const size_t words = flat.size();
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != device) (void)hipSetDevice(device);
+  int64_t* base = nullptr;
+  hipError_t e = hipMalloc(&base, words * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMemcpy(base, flat.data(), words * sizeof(int64_t), hipMemcpyHostToDevice);
+  if (prev != device) (void)hipSetDevice(prev);
+  if (e != hipSuccess) {
+    if (base) (void)hipFree(base);
+    fail(ERR_INTERNAL, std::string("uploading scatter vectors: ") + hipGetErrorString(e));
+  }
+  return dev.emplace(device, tables_at(base)).first->second;
 }
 
 ContractArgs Plan::args(int device, const void* A, const void* B, const void* C, void* D, const double alpha[2],
                         const double beta[2], bool read_c) {
-  const DeviceScatter& s = on_device(device);
+  return args_with(on_device(device), A, B, C, D, alpha, beta, read_c);
+}
+
+ContractArgs Plan::args_with(const DeviceScatter& s, const void* A, const void* B, const void* C, void* D,
+                             const double alpha[2], const double beta[2], bool read_c) const {
   ContractArgs a{};
   a.A = A;
   a.B = B;

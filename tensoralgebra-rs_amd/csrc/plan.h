@@ -65,6 +65,7 @@ struct Plan {
   // Device copies, uploaded on first use per device.
   std::mutex mu;
   std::map<int, DeviceScatter> dev;
+  std::vector<int64_t> host_image;  // host_tables()
 
   // Owners: the TAPP handle and every group built on the plan. The plan is
   // deleted when the last of them lets go (release).
@@ -80,9 +81,20 @@ struct Plan {
 
   int64_t flops() const { return (is_complex(prob.dtype) ? 8 : 2) * M * N * K * H; }
   const DeviceScatter& on_device(int device);
-  // Kernel arguments for data pointers (device addresses) and scalars.
+  // The scatter vectors as one int64 image (the layout on_device uploads):
+  // the 13 vectors back to back, then the 32-bit K offsets of A and B.
+  std::vector<int64_t> flat_tables() const;
+  // The same layout's pointers into an image at `base` (device memory).
+  DeviceScatter tables_at(int64_t* base) const;
+  // flat_tables(), built once and kept with the plan (a stable host source
+  // for copies into caller-provided device memory).
+  const std::vector<int64_t>& host_tables();
+  // Kernel arguments for data pointers (device addresses) and scalars, with
+  // the scatter vectors of this device (uploaded on first use) or of `s`.
   ContractArgs args(int device, const void* A, const void* B, const void* C, void* D, const double alpha[2],
                     const double beta[2], bool read_c);
+  ContractArgs args_with(const DeviceScatter& s, const void* A, const void* B, const void* C, void* D,
+                         const double alpha[2], const double beta[2], bool read_c) const;
   std::string kernel_name() const;
   ~Plan();
 };

@@ -65,19 +65,6 @@ Plan* adjoint(Plan* fwd, int which) {
   return a;
 }
 
-void read_scalar(const void* p, DType t, double out[2]) {
-  if (!p) fail(ERR_INVALID_ARGUMENT, "alpha is null or zero");
-  out[0] = out[1] = 0;
-  switch (t) {
-    case DType::F32: out[0] = *(const float*)p; break;
-    case DType::F64: out[0] = *(const double*)p; break;
-    case DType::C32:
-      out[0] = ((const float*)p)[0];
-      out[1] = ((const float*)p)[1];
-      break;
-    case DType::C64: memcpy(out, p, 16); break;
-  }
-}
 
 bool on_device(const void* p, int device) {
   hipPointerAttribute_t at{};
@@ -101,7 +88,7 @@ TAPP_error tprims_product_vjp(TAPP_tensor_product plan, TAPP_executor exec, cons
     if (dA && !B) fail(ERR_INVALID_ARGUMENT, "B is needed for dA");
     if (dB && !A) fail(ERR_INVALID_ARGUMENT, "A is needed for dB");
     double al[2];
-    read_scalar(alpha, fwd->prob.dtype, al);
+    read_scalar(alpha, fwd->prob.dtype, "alpha", al);
     const double alc[2] = {al[0], -al[1]};  // conj(alpha)
     const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
     Plan* adjA = dA ? adjoint(fwd, 0) : nullptr;

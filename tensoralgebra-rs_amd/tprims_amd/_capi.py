@@ -94,7 +94,7 @@ EXPORTED = [
     "tprims_tapp_executor_create_rayon", "tprims_tapp_executor_set_budget", "tprims_tapp_executor_get_threads",
     "tprims_tapp_executor_create_hip", "tprims_tapp_executor_set_async", "tprims_tapp_executor_synchronize",
     "tprims_tapp_executor_get_stream", "tprims_tapp_executor_get_device", "tprims_create_tensor_product_ex",
-    "tprims_product_report", "tprims_execute_product_dlpack", "tprims_group_create", "tprims_group_execute",
+    "tprims_product_report", "tprims_execute_product_noalloc", "tprims_execute_product_dlpack", "tprims_group_create", "tprims_group_execute",
     "tprims_group_use_graph", "tprims_group_execute_each", "tprims_group_info", "tprims_group_set_tile",
     "tprims_group_autotune", "tprims_group_destroy", "tprims_product_vjp", "tprims_product_vjp_report",
 ]
@@ -126,6 +126,7 @@ def _declare(lib):
                                             + [c_int, ctypes.POINTER(PlanConfigC)]),
         "TAPP_destroy_tensor_product": (c_int, [intptr]),
         "TAPP_execute_product": (c_int, [intptr, intptr, ctypes.POINTER(intptr), vp, vp, vp, vp, vp, vp]),
+        "tprims_execute_product_noalloc": (c_int, [intptr, intptr, ctypes.POINTER(intptr), vp, vp, vp, vp, vp, vp]),
         "TAPP_execute_batched_product": (c_int, [intptr, intptr, ctypes.POINTER(intptr), c_int, vp,
                                                  ctypes.POINTER(vp), ctypes.POINTER(vp), vp, ctypes.POINTER(vp),
                                                  ctypes.POINTER(vp)]),
