@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--corpus", action="store_true",
+                    help="time the 49-case TCCG corpus (the reference tcbench's, 64 MiB per tensor) in every dtype "
+                         "and print one JSON line: TFLOP/s per case, min/median/geomean/max per dtype")
     ap.add_argument("--only-config", default=None,
                     choices=["f64_24", "f64_gemm4096", "transform", "gemm4096", "grouped", "tapp_batched",
                              "elementwise", "vjp", "fixed_cost"],
@@ -405,6 +408,59 @@ def config_fixed_cost():
     return rec
 
 
+# The reference's published TCCG corpus figures (49 cases, 64 MiB, GF/s on one
+# Ice Lake-SP core, docs/archive/tensorprimitives/results.md:88-93).
+TCCG_REFERENCE_GFS = {"f32": {"min": 11.0, "median": 74.8, "geomean": 77.2, "max": 168.8},
+                      "f64": {"min": 7.1, "median": 44.7, "geomean": 43.7, "max": 71.6},
+                      "c32": {"min": 25.5, "median": 121.0, "geomean": 113.9, "max": 179.3},
+                      "c64": {"min": 15.7, "median": 62.0, "geomean": 63.3, "max": 91.1}}
+
+
+def corpus_run(torch, tp, ex, stream, steps, warmup):
+    """Every TCCG case (tests/tccg.py) in every dtype at 64 MiB per tensor:
+    inputs N(0,1) resident in HBM, the planner's default plan, HIP events on
+    the executor's stream."""
+    import math
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import tccg
+
+    dts = [("f32", tp.DType.F32, torch.float32), ("f64", tp.DType.F64, torch.float64),
+           ("c32", tp.DType.C32, torch.complex64), ("c64", tp.DType.C64, torch.complex128)]
+    rows, summary = [], {}
+    for name, DT, tdt in dts:
+        vals = []
+        for case in tccg.corpus():
+            s = tccg.Sized(case, 64 * 1024 * 1024)
+            spec = lambda x: tp.OperandSpec(tp.LayoutSpec(*s.layout(x)))
+            lab = lambda x: [ord(ch) for ch in x]
+            prob = tp.Problem.from_labels(DT, spec(s.a), spec(s.b), tp.CSpec.separate(spec(s.c)), spec(s.c),
+                                          tp.Labels(lab(s.a), lab(s.b), lab(s.c), lab(s.c)))
+            plan = tp.Plan(prob)
+            g = torch.Generator(device="cuda").manual_seed(1)
+            A = torch.randn(s.storage(s.a), dtype=tdt, device="cuda", generator=g)
+            B = torch.randn(s.storage(s.b), dtype=tdt, device="cuda", generator=g)
+            D = torch.empty(s.storage(s.c), dtype=tdt, device="cuda")
+            fn = lambda: plan.execute_raw(1.0, A.data_ptr(), B.data_ptr(), 0.0, None, D.data_ptr(), ex)
+            ms = timed(torch, fn, stream, steps, warmup)
+            flops = (8 if DT.is_complex else 2) * s.macs()
+            tf = flops / (ms * 1e-3) / 1e12
+            vals.append(tf)
+            rows.append({"case": case[0], "group": case[1], "dtype": name, "ms": round(ms, 4),
+                         "tflops": round(tf, 3), "kernel": plan.report()["kernel"]})
+            del A, B, D, plan
+        vals.sort()
+        med = vals[len(vals) // 2] if len(vals) % 2 else 0.5 * (vals[len(vals) // 2 - 1] + vals[len(vals) // 2])
+        geo = math.exp(sum(math.log(v) for v in vals) / len(vals))
+        summary[name] = {"min": round(vals[0], 3), "median": round(med, 3), "geomean": round(geo, 3),
+                         "max": round(vals[-1], 3),
+                         "reference_one_core_gflops": TCCG_REFERENCE_GFS[name]}
+        torch.cuda.empty_cache()
+        print("bench: corpus %s done" % name, file=sys.stderr, flush=True)
+    return {"metric": "TCCG corpus TFLOP/s (49 cases, 64 MiB per tensor)", "unit": "TFLOP/s", "steps": steps,
+            "warmup": warmup, "summary": summary, "cases": rows}
+
+
 def cpu_baseline(args):
     """The C++ port of the reference's packed driver on the host cores, timed
     on one batch entry (n = 1 of 16) of the headline contraction, on every
@@ -450,6 +506,9 @@ def main():
     table = config_table(torch, tp, np, ex, stream, args.steps)
     if args.only_config:
         print(json.dumps({"config": args.only_config, **dict(table)[args.only_config]()}))
+        return
+    if args.corpus:
+        print(json.dumps(corpus_run(torch, tp, ex, stream, args.steps, min(args.warmup, 3))))
         return
     w = Complex(torch, tp, args.dtype, args.method, args.precision)
     t0 = time.perf_counter()

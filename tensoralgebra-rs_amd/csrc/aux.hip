@@ -34,8 +34,9 @@ __global__ void k_output_pass(ContractArgs p) {
       h = r / p.M;
     }
     S* D = (S*)p.D;
-    const int64_t off = p.hd[h] + p.dm[m] + p.dn[n];
-    const int64_t coff = p.read_c ? p.hc[h] + p.cm[m] + p.cn[n] : 0;
+    const int64_t off = p.hd[h] + (p.mn_affine ? m * p.cd_stride[2] + n * p.cd_stride[3] : p.dm[m] + p.dn[n]);
+    const int64_t coff =
+        !p.read_c ? 0 : p.hc[h] + (p.mn_affine ? m * p.cd_stride[0] + n * p.cd_stride[1] : p.cm[m] + p.cn[n]);
     // Broadcast copies (adjoints) read their own previous value: C is D there.
     const int64_t copies = p.dr == nullptr ? 1 : p.R;
     for (int64_t r = 0; r < copies; r++) {

@@ -138,8 +138,9 @@ __device__ inline void store_element(const ContractArgs& p, int64_t h, int64_t m
   const int64_t oc = p.read_c ? (p.h_affine ? h * p.h_stride[2] : p.hc[h]) : 0;
   const int64_t od = p.h_affine ? h * p.h_stride[3] : p.hd[h];
   const T vr = alr * sr - ali * si, vi = alr * si + ali * sr;
-  S* D = (S*)p.D + od + p.dm[m] + p.dn[n];
-  const int64_t coff = p.read_c ? oc + p.cm[m] + p.cn[n] : 0;
+  S* D = (S*)p.D + od + (p.mn_affine ? m * p.cd_stride[2] + n * p.cd_stride[3] : p.dm[m] + p.dn[n]);
+  const int64_t coff =
+      !p.read_c ? 0 : oc + (p.mn_affine ? m * p.cd_stride[0] + n * p.cd_stride[1] : p.cm[m] + p.cn[n]);
   const int64_t copies = p.dr == nullptr ? 1 : p.R;
   for (int64_t r = 0; r < copies; r++) {
     const int64_t roff = p.dr == nullptr ? 0 : p.dr[r];
@@ -188,8 +189,11 @@ __device__ inline void write_back(const ContractArgs& p, int64_t h, int64_t m0, 
     T xi = CPLX ? (T)Dim[ml * LDD + nl] : T(0);
     const T pr = alr * xr - ali * xi;
     const T pi = alr * xi + ali * xr;
-    const int64_t off = p.dm[m] + p.dn[n];
-    const int64_t coff = rc ? p.cm[m] + p.cn[n] : 0;
+    // One folded axis per role: affine offsets, no scatter-vector loads (a
+    // latency chain per element otherwise; the f32 transform step's 16-slab
+    // tiles spent 40 % of their time here).
+    const int64_t off = p.mn_affine ? m * p.cd_stride[2] + n * p.cd_stride[3] : p.dm[m] + p.dn[n];
+    const int64_t coff = !rc ? 0 : p.mn_affine ? m * p.cd_stride[0] + n * p.cd_stride[1] : p.cm[m] + p.cn[n];
     // A broadcast (adjoint) writes every copy; its C, when read, is D itself,
     // so each copy accumulates its own previous value.
     const int64_t copies = p.dr == nullptr ? 1 : R;

@@ -476,6 +476,7 @@ bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double 
   a.a_kvec = a.a_kvec && a16;
   a.b_kvec = a.b_kvec && b16;
   a.ab_vec = a.ab_vec && a16 && b16;  // every item's origin, not only the first
+  a.a_vec16 = a.a_vec16 && a16;
   // The items multiply the tile count: K is split only while the whole batch
   // would still leave the GPU underfilled.
   int ksplit = p.ksplit;

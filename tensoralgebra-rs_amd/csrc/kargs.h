@@ -52,6 +52,10 @@ struct ContractArgs {
   // extent there is a multiple of 16 / es, and every run start, k offset,
   // row offset and batch offset is a multiple of 16 bytes, as are A and B.
   int32_t ab_vec;
+  // A alone gathers in such 16-byte runs (off32, f32/f64/c32; its base
+  // 16-byte aligned): what the split-BF16 kernel's in-kernel split of an
+  // M-major A needs.
+  int32_t a_vec16;
   // With off32: the K scatters as 32-bit byte offsets, entries from K up to a
   // multiple of 64 holding the out-of-range sentinel 0x40000000.
   const uint32_t* ak32;
@@ -60,6 +64,11 @@ struct ContractArgs {
   // C, D] (used by the elementwise kernel instead of the scatter vectors).
   int64_t h_stride[4];
   int32_t h_affine;
+  // The M and N roles are each at most one folded axis: C and D offsets are
+  // m * cd_stride[0] + n * cd_stride[1] (C) and m * cd_stride[2] + n *
+  // cd_stride[3] (D), with no scatter-vector loads in the write-back.
+  int32_t mn_affine;
+  int64_t cd_stride[4];
   // Split K: ksplit workgroups per output tile, each over kslabs K slabs,
   // writing its unscaled partial tile to work[split][h][m][n] (storage type);
   // a reduction pass sums the splits in order and does the write-back.

@@ -47,6 +47,9 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int t
 // Split-BF16 for f32/c32 storage (nterms 2 or 3): A and B are split once into
 // bf16 fragment panels (device scratch of split_panel_bytes), then multiplied.
 size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id);
+// Whether split-BF16 splits A inside the GEMM rather than packing A panels
+// (with 32-bit gather offsets and a 16-byte aligned A).
+bool split_a_in_gemm(bool cplx, bool a_kmajor, bool a_vec16, int64_t N, int tile_id);
 hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int tile_id, void* panels,
                                  hipStream_t s);
 // FP8 (e4m3, one power-of-two scale per 128-k panel of every row) for f32/c32

@@ -210,8 +210,26 @@ void configure(Plan* plan, const PlanConfig& cfg) {
     const Axis &xa = p.k_ax[kA], &xb = p.k_ax[kB];
     int64_t lineA = std::max<int64_t>(1, 128 / std::max<int64_t>(1, (int64_t)uabs(xa.sa) * es));
     int64_t lineB = std::max<int64_t>(1, 128 / std::max<int64_t>(1, (int64_t)uabs(xb.sb) * es));
-    int64_t aLo = largest_divisor_le(xa.extent, std::min(BK, lineA));
-    int64_t bLo = largest_divisor_le(xb.extent, std::max<int64_t>(1, BK / aLo));
+    // One slab of BK k holds aLo consecutive indices of A's contiguous K axis
+    // times bLo of B's: A reads runs of aLo elements, B runs of bLo. Of the
+    // divisor pairs with aLo * bLo <= BK, take the one whose worse operand
+    // uses the larger share of its cache line, then the fuller slab (extents
+    // 72 x 72 in f64: 4 x 4, where taking A's longest run first gives 12 x 1
+    // and one line per element of B).
+    int64_t aLo = 1, bLo = 1;
+    double best_share = -1.0;
+    for (int64_t da = 1; da <= std::min(BK, xa.extent); da++) {
+      if (xa.extent % da != 0) continue;
+      for (int64_t db = 1; da * db <= BK && db <= xb.extent; db++) {
+        if (xb.extent % db != 0) continue;
+        const double share = std::min((double)std::min(da, lineA) / lineA, (double)std::min(db, lineB) / lineB);
+        if (share > best_share + 1e-12 || (share > best_share - 1e-12 && da * db > aLo * bLo)) {
+          best_share = share;
+          aLo = da;
+          bLo = db;
+        }
+      }
+    }
     int64_t bMid = largest_divisor_le(xb.extent / bLo, std::max<int64_t>(1, lineB / bLo));
     push_nonunit(korder, scaled(xa, aLo, 1));
     push_nonunit(korder, scaled(xb, bLo, 1));
@@ -310,6 +328,7 @@ void configure(Plan* plan, const PlanConfig& cfg) {
   // scatter terms: the major scatter is regular with stride 1 at block size
   // q, with no ragged last block.
   p.ab_vec = false;
+  p.a_vec16 = false;
   if (p.off32 && es < 16) {  // f32, f64, c32 (a c64 element is already one 16-byte DMA)
     const int64_t q = 16 / es;
     auto runs = [&](const std::vector<int64_t>& v, int64_t n) {
@@ -337,8 +356,16 @@ void configure(Plan* plan, const PlanConfig& cfg) {
         if (rv[i] % q != 0) return false;
       return true;
     };
-    p.ab_vec = operand(p.a_kmajor, p.am, p.M, p.ak, p.ha) && operand(p.b_kmajor, p.bn, p.N, p.bk, p.hb);
+    p.a_vec16 = operand(p.a_kmajor, p.am, p.M, p.ak, p.ha);
+    p.ab_vec = p.a_vec16 && operand(p.b_kmajor, p.bn, p.N, p.bk, p.hb);
   }
+  // Split-BF16 that splits A inside the GEMM runs its 128x128 tile (two
+  // workgroups per CU, whose splitting and MFMAs overlap) rather than
+  // 256x128 (one workgroup per CU): f32 transform step, 3 terms, 135 -> 157
+  // TFLOP/s. Both tiles have 32-k steps, so the K order above stands.
+  if (cfg.tile == 0 && p.tile == STILE_256x128 && p.off32 &&
+      split_a_in_gemm(false, p.a_kmajor, p.a_vec16, p.N, STILE_128x128))
+    p.tile = STILE_128x128;
 }
 
 std::vector<int64_t> build_block_scatter(const int64_t* scat, int64_t len, int64_t blk) {
@@ -498,8 +525,16 @@ ContractArgs Plan::args_with(const DeviceScatter& s, const void* A, const void* 
   a.b_kvec = b_kvec && (uintptr_t)B % 16 == 0;
   a.off32 = off32;
   a.ab_vec = ab_vec && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0;
+  a.a_vec16 = a_vec16 && (uintptr_t)A % 16 == 0;
   a.ak32 = s.ak32;
   a.bk32 = s.bk32;
+  a.mn_affine = m_ax.size() <= 1 && n_ax.size() <= 1;
+  if (a.mn_affine) {
+    a.cd_stride[0] = m_ax.empty() ? 0 : m_ax[0].sc;
+    a.cd_stride[1] = n_ax.empty() ? 0 : n_ax[0].sc;
+    a.cd_stride[2] = m_ax.empty() ? 0 : m_ax[0].sd;
+    a.cd_stride[3] = n_ax.empty() ? 0 : n_ax[0].sd;
+  }
   a.h_affine = h_ax.size() == 1;
   if (a.h_affine) {
     a.h_stride[0] = h_ax[0].sa;
@@ -530,6 +565,11 @@ std::string Plan::kernel_name() const {
   if (cfg.precision == PREC_STORAGE && !fallback && dma_supported(off32, prob.dtype, cfg.complex_method, tile))
     s += ab_vec ? ".dma16" : ".dma";
   if (ksplit > 1) s += ".splitk" + std::to_string(ksplit);
+  // Split-BF16 with an f32 A gathered along M in 16-byte runs: A is split
+  // inside the GEMM (for a 16-byte aligned A, checked per call).
+  if ((cfg.precision == PREC_SPLIT_BF16_2 || cfg.precision == PREC_SPLIT_BF16_3) && off32 &&
+      split_a_in_gemm(is_complex(prob.dtype), a_kmajor, a_vec16, N, tile))
+    s += ".a_in_gemm";
   return s;
 }
 

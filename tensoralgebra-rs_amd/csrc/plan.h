@@ -49,6 +49,7 @@ struct Plan {
   bool a_kvec = false, b_kvec = false;
   bool off32 = false;  // ContractArgs::off32
   bool ab_vec = false;  // host half of ContractArgs::ab_vec (pointer alignment checked per call)
+  bool a_vec16 = false;  // host half of ContractArgs::a_vec16
   // Split K (ContractArgs::ksplit, kslabs): 1, or the number of K ranges a
   // tile's work is divided into when the tiles alone would leave most of the
   // GPU idle; the partials are reduced in a fixed order.

@@ -59,6 +59,28 @@ __device__ inline void split(float x, __bf16* t) {
   }
 }
 
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+
+// The same split for two values at once, each term as the packed dword of
+// the pair (x0 in the low half), which is what the MFMA operand holds: one
+// v_cvt_pk_bf16_f32 per term and pair, the terms' f32 values recovered by
+// shifting or masking the packed halves.
+template <int TERMS>
+__device__ inline void split_pair(float x0, float x1, uint32_t* t) {
+  const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){x0, x1}, v2bf));
+  const float r0 = x0 - __builtin_bit_cast(float, h << 16);
+  const float r1 = x1 - __builtin_bit_cast(float, h & 0xffff0000u);
+  const uint32_t m = __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){r0, r1}, v2bf));
+  t[0] = h;
+  t[1] = m;
+  if constexpr (TERMS == 3) {
+    const float s0 = r0 - __builtin_bit_cast(float, m << 16);
+    const float s1 = r1 - __builtin_bit_cast(float, m & 0xffff0000u);
+    t[2] = __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){s0, s1}, v2bf));
+  }
+}
+
 struct PackArgs {
   const void* X;
   const int64_t* rv;  // row scatter (length R)
@@ -212,9 +234,16 @@ __global__ __launch_bounds__(256) void k_pack_lds(PackArgs p) {
   }
 }
 
-template <bool CPLX_, int TERMS_, int BM_, int BN_, int WM_, int WN_, int STAGES_>
+// AF_: A is not packed; it is an f32 operand gathered along M in 16-byte
+// runs (ContractArgs::a_vec16 with A M-major). Each K step DMAs A's BM x 32
+// f32 block straight into LDS (buffer_load ... lds, 16 bytes per lane), and
+// every lane splits the 8 values it feeds the MFMA as it reads them. The A
+// block is [32 k][BM m] with the 16-byte chunks of row k XOR-permuted by
+// 4 ((k / 8) % 4): a fragment read (16 rows x 8 k per lane group, the four
+// lane groups 8 k apart) then hits 64 distinct banks.
+template <bool CPLX_, int TERMS_, int BM_, int BN_, int WM_, int WN_, int STAGES_, bool AF_ = false>
 struct GCfg {
-  static constexpr bool CPLX = CPLX_;
+  static constexpr bool CPLX = CPLX_, AF = AF_;
   static constexpr int TERMS = TERMS_, STAGES = STAGES_;
   static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;
   static constexpr int WAVES_M = BM / WM;
@@ -222,16 +251,24 @@ struct GCfg {
   static constexpr int NT = NW * 64;
   static constexpr int COMPS = CPLX ? 2 : 1;
   static constexpr int PER = COMPS * TERMS;
-  static constexpr int A_FRAGS = BM / 16 * PER, B_FRAGS = BN / 16 * PER;
+  static constexpr int A_FRAGS = AF ? 0 : BM / 16 * PER, B_FRAGS = BN / 16 * PER;
   static constexpr int FRAGS = A_FRAGS + B_FRAGS;  // 1 KiB fragments per K step
   static constexpr int PW = FRAGS / NW;            // fragments each wave copies per K step
+  // The raw A block (AF): 16-byte chunks, 64 per DMA instruction.
+  static constexpr int A_CHUNKS_ROW = BM / 4;
+  static constexpr int A_INSTR = AF ? KBLK * A_CHUNKS_ROW / 64 : 0;
+  static constexpr int AW = A_INSTR / NW;          // A instructions each wave issues per K step
+  static constexpr int IW = PW + AW;               // every copy instruction of a wave per K step
+  static constexpr size_t A_RAW_BYTES = AF ? (size_t)KBLK * BM * 4 : 0;
   static constexpr int MI = WM / 16, NI = WN / 16;
   static constexpr int LDD = BN + 1;
-  static constexpr size_t STAGE_BYTES = (size_t)FRAGS * FRAG * 2;
+  static constexpr size_t STAGE_BYTES = A_RAW_BYTES + (size_t)FRAGS * FRAG * 2;
   static constexpr size_t LDS_MAIN = STAGES * STAGE_BYTES;
   static constexpr size_t LDS_EPI = (size_t)COMPS * BM * LDD * 4;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
   static_assert(FRAGS % NW == 0, "whole fragments per wave");
+  static_assert(!AF || (!CPLX && A_INSTR % NW == 0 && A_CHUNKS_ROW % 16 == 0),
+                "raw A: f32, whole instructions per wave, room for the chunk XOR");
   static_assert(LDS <= 160 * 1024, "LDS budget");
 };
 
@@ -244,10 +281,16 @@ __device__ inline void wait_vmcnt() {
 
 struct GemmArgs {
   ContractArgs c;
-  const __bf16* Ap;
+  const __bf16* Ap;  // unused when A is split in the kernel
   const __bf16* Bp;
   int64_t RBa, RBb, KB;
 };
+
+// LDS float index of raw A element (row m, k) within a stage's A block.
+template <int BM>
+__device__ inline int araw_slot(int m, int k) {
+  return k * BM + ((((m >> 2) ^ (4 * ((k >> 3) & 3)))) << 2) + (m & 3);
+}
 
 template <class G>
 __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
@@ -271,10 +314,39 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
   const __bf16* Bg = g.Bp + (h * g.KB * g.RBb + (int64_t)tn * (BN / 16)) * PER * FRAG;
   const int64_t a_step = g.RBa * PER * FRAG;  // bf16 per K block
   const int64_t b_step = g.RBb * PER * FRAG;
+  // Raw A (AF): instruction t of this wave moves flat chunks
+  // (wave + NW t) * 64 + lane of the [32 k][BM / 4 chunk] block; the chunk at
+  // position x of row k holds rows 4 (x ^ 4 ((k / 8) % 4)) .. + 3.
+  [[maybe_unused]] const __amdgpu_buffer_rsrc_t rsA = dev::make_rsrc(G::AF ? (const void*)((const float*)p.A + p.ha[h]) : p.A);
+  uint32_t a_row[G::AF ? G::AW : 1];
+  int a_kl[G::AF ? G::AW : 1];
+  if constexpr (G::AF) {
+#pragma unroll
+    for (int t = 0; t < G::AW; t++) {
+      const int f = (wave + G::NW * t) * 64 + lane;
+      const int k = f / G::A_CHUNKS_ROW, x = f % G::A_CHUNKS_ROW;
+      const int64_t m = (int64_t)tm * BM + 4 * (x ^ (4 * ((k >> 3) & 3)));
+      a_row[t] = m < p.M ? (uint32_t)(p.am[m] * 4) : dev::ROW_OOB;
+      a_kl[t] = k;
+    }
+  }
   // Copies K block kb into LDS stage `st`, straight from global memory: one
-  // instruction per fragment, lane l moving bytes [16 l, 16 l + 16).
+  // instruction per fragment, lane l moving bytes [16 l, 16 l + 16); with a
+  // raw A its block first, then B's fragments.
   auto issue = [&](int64_t kb, int st) {
     char* stage = smem + (size_t)st * G::STAGE_BYTES;
+    if constexpr (G::AF) {
+#pragma unroll
+      for (int t = 0; t < G::AW; t++) {
+        [[maybe_unused]] const uint32_t off = a_row[t] + p.ak32[kb * KBLK + a_kl[t]];
+#if defined(__HIP_DEVICE_COMPILE__)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rsA, (__attribute__((address_space(3))) void*)(stage + (size_t)(wave + G::NW * t) * 1024), 16, off, 0, 0,
+            0);
+#endif
+      }
+      stage += G::A_RAW_BYTES;
+    }
 #pragma unroll
     for (int i = 0; i < G::PW; i++) {
       const int f = wave + i * G::NW;  // wave-uniform fragment index
@@ -303,6 +375,29 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
     return *(const v8bf*)((const __bf16*)stage + ((size_t)(blk * PER + c * G::TERMS + x) * FRAG + lane * 8));
   };
   v8bf a[G::COMPS][G::TERMS][G::MI], b[G::COMPS][G::TERMS][G::NI], nb[G::COMPS][G::TERMS][G::NI];
+  // Raw A (AF): the 8 f32 values lane (row lane % 16, k group lane / 16) feeds
+  // the MFMA for row block i of a stage, and their split into a[0][*][i].
+  const int ar = lane & 15, ag = lane >> 4;
+  auto raw_read = [&](const char* stage, int i, float* v) {
+    const float* As = (const float*)stage;
+    const int m = wm + 16 * i + ar;
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = As[araw_slot<BM>(m, 8 * ag + j)];
+  };
+  auto raw_split = [&](const float* v, int i) {
+    v4u w[G::TERMS];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      uint32_t t[3];
+      split_pair<G::TERMS>(v[2 * q], v[2 * q + 1], t);
+#pragma unroll
+      for (int x = 0; x < G::TERMS; x++) w[x][q] = t[x];
+    }
+#pragma unroll
+    for (int x = 0; x < G::TERMS; x++) a[0][x][i] = __builtin_bit_cast(v8bf, w[x]);
+  };
+  // B's fragments of a stage start after the A block (raw) or A's fragments.
+  constexpr size_t B_OFF = G::AF ? G::A_RAW_BYTES : (size_t)G::A_FRAGS * FRAG * 2;
 
   // Two stages; K block kb lives in stage kb & 1. Registers are one block
   // ahead of the MFMAs: while block kb is multiplied, block kb + 1 is read
@@ -316,7 +411,7 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
   issue(0, 0);
   if (nk > 1) {
     issue(1, 1);
-    wait_vmcnt<G::PW>();
+    wait_vmcnt<G::IW>();
   } else {
     wait_vmcnt<0>();
   }
@@ -325,11 +420,21 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
   for (int c = 0; c < G::COMPS; c++)
 #pragma unroll
     for (int x = 0; x < G::TERMS; x++) {
+      if constexpr (!G::AF) {
 #pragma unroll
-      for (int i = 0; i < G::MI; i++) a[c][x][i] = frag(smem, (wm >> 4) + i, c, x);
+        for (int i = 0; i < G::MI; i++) a[c][x][i] = frag(smem, (wm >> 4) + i, c, x);
+      }
 #pragma unroll
-      for (int j = 0; j < G::NI; j++) b[c][x][j] = frag(smem + G::A_FRAGS * FRAG * 2, (wn >> 4) + j, c, x);
+      for (int j = 0; j < G::NI; j++) b[c][x][j] = frag(smem + B_OFF, (wn >> 4) + j, c, x);
     }
+  float rv[2][8];
+  if constexpr (G::AF) {
+#pragma unroll
+    for (int i = 0; i < G::MI; i++) {
+      raw_read(smem, i, rv[0]);
+      raw_split(rv[0], i);
+    }
+  }
   for (int64_t kb = 0; kb < nk; kb++) {
     // vmcnt(0) and lgkmcnt(0): own copies landed, own LDS reads complete.
     __builtin_amdgcn_s_waitcnt(7 << 4);
@@ -337,7 +442,7 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
     if (kb + 2 < nk) issue(kb + 2, (int)(kb & 1));
     // Block kb + 1 (stale data in the last iteration, never used).
     const char* An = smem + (size_t)((kb + 1) & 1) * G::STAGE_BYTES;
-    const char* Bn = An + G::A_FRAGS * FRAG * 2;
+    const char* Bn = An + B_OFF;
 #pragma unroll
     for (int c = 0; c < G::COMPS; c++)
 #pragma unroll
@@ -362,12 +467,20 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
                                                                         acc[slot][i][j], 0, 0, 0);
         }
       __builtin_amdgcn_sched_barrier(0);
+      if constexpr (G::AF) {
+        // Row block i of block kb + 1: its values are read now and split
+        // after the next row block's MFMAs (the LDS latency hides under them).
+        if (i > 0) raw_split(rv[(i - 1) & 1], i - 1);
+        raw_read(An, i, rv[i & 1]);
+      } else {
 #pragma unroll
-      for (int c = 0; c < G::COMPS; c++)
+        for (int c = 0; c < G::COMPS; c++)
 #pragma unroll
-        for (int x = 0; x < G::TERMS; x++) a[c][x][i] = frag(An, (wm >> 4) + i, c, x);
+          for (int x = 0; x < G::TERMS; x++) a[c][x][i] = frag(An, (wm >> 4) + i, c, x);
+      }
       __builtin_amdgcn_sched_barrier(0);
     }
+    if constexpr (G::AF) raw_split(rv[(G::MI - 1) & 1], G::MI - 1);
 #pragma unroll
     for (int c = 0; c < G::COMPS; c++)
 #pragma unroll
@@ -414,9 +527,15 @@ void launch_gemm(const GemmArgs& g, int64_t nwg, hipStream_t s) {
   hipLaunchKernelGGL(k_gemm<G>, dim3((unsigned)nwg), dim3(G::NT), 0, s, g);
 }
 
+bool split_in_gemm(const ContractArgs& a, bool cplx, int tile_id) {
+  return a.off32 && a.ak32 != nullptr &&
+         split_a_in_gemm(cplx, a.a_kmajor, a.a_vec16, a.N, tile_id);
+}
+
 template <bool CPLX, int TERMS>
 hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) {
   constexpr int PER = (CPLX ? 2 : 1) * TERMS;
+  const bool fuse_a = split_in_gemm(a, CPLX, tile_id);
   const Shape sh = shape_of(tile_id, CPLX);
   const int64_t KB = (a.K + KBLK - 1) / KBLK;
   const int64_t tiles_m = (a.M + sh.bm - 1) / sh.bm, tiles_n = (a.N + sh.bn - 1) / sh.bn;
@@ -427,10 +546,11 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
   };
   const int64_t RBa = pad(a.M, sh.bm), RBb = pad(a.N, sh.bn);
   __bf16* Ap = (__bf16*)panels;
-  __bf16* Bp = Ap + a.H * KB * RBa * PER * FRAG;
+  __bf16* Bp = fuse_a ? Ap : Ap + a.H * KB * RBa * PER * FRAG;
   PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kvec, a.a_kmajor, Ap};
   PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, a.b_kmajor, Bp};
   for (const PackArgs* q : {&pa, &pb}) {
+    if (q == &pa && fuse_a) continue;
     const int64_t grid = q->H * q->KB * (q->RB / 4);
     if (grid > max_grid(256)) return hipErrorInvalidConfiguration;
     if (q->vec) hipLaunchKernelGGL((k_pack<CPLX, TERMS, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
@@ -449,6 +569,12 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
   const int64_t nwg = tiles_m * tiles_n * a.H;
   if (nwg > max_grid(512)) return hipErrorInvalidConfiguration;
   if constexpr (!CPLX) {
+    if (fuse_a) {
+      if (sh.bm == 256) launch_gemm<GCfg<CPLX, TERMS, 256, 128, 64, 64, 2, true>>(g, nwg, s);
+      else if (sh.bm == 128) launch_gemm<GCfg<CPLX, TERMS, 128, 128, 64, 64, 2, true>>(g, nwg, s);
+      else launch_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32, 2, true>>(g, nwg, s);
+      return hipGetLastError();
+    }
     if (sh.bm == 256) launch_gemm<GCfg<CPLX, TERMS, 256, 128, 64, 64, 2>>(g, nwg, s);
     else if (sh.bm == 128) launch_gemm<GCfg<CPLX, TERMS, 128, 128, 64, 64, 2>>(g, nwg, s);
     else launch_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32, 2>>(g, nwg, s);
@@ -461,6 +587,17 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
 
 }  // namespace
 
+// An f32 A gathered along M in 16-byte runs is split inside the GEMM (no A
+// panels): the transform step's 537 MB A was otherwise read, written as 805 MB
+// of panels and read again. Each of a tile column's workgroups splits its own
+// copy, so wide products (many tile columns) keep the panels, which split
+// every element once (transform step, N = 512: 100.9 -> 115 TFLOP/s; the
+// 4096^3 GEMM, 32 tile columns: 170 with panels against 150).
+bool split_a_in_gemm(bool cplx, bool a_kmajor, bool a_vec16, int64_t N, int tile_id) {
+  const int64_t bn = shape_of(tile_id, cplx).bn;
+  return !cplx && !a_kmajor && a_vec16 && (N + bn - 1) / bn <= 8;
+}
+
 size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id) {
   const bool cplx = t == DType::C32;
   const Shape sh = shape_of(tile_id, cplx);
@@ -470,7 +607,8 @@ size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id
     return (rb + 3) / 4 * 4;
   };
   const int64_t per = (cplx ? 2 : 1) * nterms;
-  return (size_t)a.H * KB * (pad(a.M, sh.bm) + pad(a.N, sh.bn)) * per * FRAG * 2;
+  const int64_t rows = (split_in_gemm(a, cplx, tile_id) ? 0 : pad(a.M, sh.bm)) + pad(a.N, sh.bn);
+  return (size_t)a.H * KB * rows * per * FRAG * 2;
 }
 
 hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int tile_id, void* panels,
