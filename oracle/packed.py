@@ -40,9 +40,11 @@ def lib():
         out = os.path.join(HERE, "_build", "packed_%s_%s.so" % (_cpu_tag(), tag))
         if not os.path.exists(out):
             os.makedirs(os.path.dirname(out), exist_ok=True)
+            # One temporary name per process: parallel test workers may compile at the same time.
+            tmp = "%s.tmp%x" % (out, os.getpid())
             subprocess.check_call(["g++", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
-                                   "-o", out + ".tmp", src])
-            os.replace(out + ".tmp", out)
+                                   "-o", tmp, src])
+            os.replace(tmp, out)
         l = ctypes.CDLL(out)
         i64p = ctypes.POINTER(ctypes.c_int64)
         dp = ctypes.POINTER(ctypes.c_double)

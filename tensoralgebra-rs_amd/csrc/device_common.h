@@ -86,11 +86,15 @@ struct Store<T, true> {
 };
 
 // Workgroup id -> (h, tile_m, tile_n). Consecutive hardware ids go round-robin
-// over the 8 XCDs; remapping gives each XCD a contiguous run of logical tiles,
-// and the logical order walks groups of 8 tile-rows so neighbours share A rows
-// and B columns in that XCD's L2.
+// over the 8 XCDs; remapping gives each XCD a contiguous run of logical tiles
+// (XCD x of id b = 8 i + x takes run x: the first nwg % 8 runs are one tile
+// longer), and the logical order walks groups of 8 tile-rows so neighbours
+// share A rows and B columns in that XCD's L2.
 __device__ inline void decode_tile(int64_t bid, int64_t nwg, int tiles_m, int tiles_n, int64_t& h, int& tm, int& tn) {
-  if ((nwg & 7) == 0) bid = (bid & 7) * (nwg >> 3) + (bid >> 3);
+  {
+    const int64_t q = nwg >> 3, r = nwg & 7, x = bid & 7, i = bid >> 3;
+    bid = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+  }
   const int64_t per_h = (int64_t)tiles_m * tiles_n;
   h = bid / per_h;
   int t = (int)(bid - h * per_h);

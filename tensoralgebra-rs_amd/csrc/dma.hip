@@ -64,8 +64,13 @@ __device__ inline void dma_load(__amdgpu_buffer_rsrc_t r, void* dst, uint32_t of
 // elements consecutive in memory along the operand's major index (the
 // planner's ContractArgs::ab_vec); a quarter of the DMA instructions of the
 // 4-byte form for f32, an eighth for f64 and c32 (no word or re/im planes).
+// KG_: K groups. The workgroup's waves form KG groups of NW waves; group g
+// computes the whole BM x BN tile over slabs g, g + KG, g + 2 KG, ... (each
+// group with its own LDS part of every stage), and the groups' accumulators
+// are summed through LDS in group order before the write-back. A small tile
+// then keeps every SIMD of its CU busy without a split-K workspace.
 template <typename T, bool CPLX_, int BM_, int BN_, int BK_, int WM_, int WN_, bool AKM_, bool BKM_, int METH_,
-          bool PLR_ = true, int MINW_ = 1, bool VEC_ = false>
+          bool PLR_ = true, int MINW_ = 1, bool VEC_ = false, int KG_ = 1>
 struct DCfg {
   typedef T Tv;
   static constexpr bool CPLX = CPLX_, AKM = AKM_, BKM = BKM_, PLR = PLR_;
@@ -73,8 +78,9 @@ struct DCfg {
   static constexpr int METH = CPLX_ ? METH_ : CPLX_NATIVE;
   static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
   static constexpr int WAVES_M = BM / WM;
-  static constexpr int NW = (BM / WM) * (BN / WN);
-  static constexpr int NT = NW * 64;
+  static constexpr int KG = KG_;
+  static constexpr int NW = (BM / WM) * (BN / WN);  // waves of one K group
+  static constexpr int NT = NW * KG * 64;
   typedef typename Store<T, CPLX>::S S;
   static constexpr int ES = sizeof(S);  // element bytes in memory
   // 8-byte elements (no DMA size) arrive as two 4-byte DMAs into separate
@@ -100,7 +106,8 @@ struct DCfg {
   static constexpr int A_ELEMS = VEC ? BM * BK : AKM ? (BM / RPI) * (64 + PADB) : BK * (BM + PADM);
   static constexpr int B_ELEMS = VEC ? BN * BK : BKM ? (BN / RPI) * (64 + PADB) : BK * (BN + PADM);
   static constexpr int PLANE = A_ELEMS + B_ELEMS;  // LDS elements of one plane
-  static constexpr int STAGE = PLANES * PLANE;     // LDS elements of one stage
+  static constexpr int STAGE = PLANES * PLANE;     // LDS elements of one K group's part of a stage
+  static constexpr int SSTRIDE = KG * STAGE;       // LDS elements of one stage
   static constexpr int A_INSTR = BM * BK / (64 * V), B_INSTR = BN * BK / (64 * V);
   static constexpr int JA = A_INSTR / NW, JB = B_INSTR / NW;  // DMA instructions per wave per slab
   static constexpr int MI = WM / 16, NI = WN / 16;
@@ -115,7 +122,7 @@ struct DCfg {
   static constexpr bool C3 = CPLX && sizeof(T) == 4 && METH == CPLX_NATIVE;
   static constexpr int ACCS = !CPLX || METH == CPLX_1M ? 1 : (C3 || METH == CPLX_3M ? 3 : 2);
   static constexpr int LDD = BN + 1;
-  static constexpr size_t LDS_MAIN = (size_t)2 * STAGE * EW;
+  static constexpr size_t LDS_MAIN = (size_t)2 * SSTRIDE * EW;
   // The finished tile is staged whole when it fits in the main loop's LDS,
   // otherwise one wave row (WM rows) at a time.
   static constexpr size_t EPI_ROW_BYTES = (size_t)(CPLX ? 2 : 1) * LDD * sizeof(T);
@@ -123,11 +130,15 @@ struct DCfg {
   static constexpr size_t LDS_EPI = (size_t)EPI_ROWS * EPI_ROW_BYTES;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
   static_assert(DMAB == 4 || DMAB == 16, "LDS DMA moves 4 or 16 bytes per lane");
-  static_assert(!VEC || (BM % 64 == 0 && BN % 64 == 0 && BM / V >= 4 * XQ && BN / V >= 4 * XQ && BK % V == 0),
-                "run layout: whole 64-row instructions, room for the XOR");
+  // Run layout: whole 64-lane instructions; an M/N-major part needs a
+  // multiple of 16 chunks per k so that x ^ XQ * (k % XR) stays in the row.
+  static_assert(!VEC || ((BM * BK / V) % 64 == 0 && (BN * BK / V) % 64 == 0 && BK % V == 0 &&
+                         (AKM || (BM / V) % 16 == 0) && (BKM || (BN / V) % 16 == 0)),
+                "run layout: whole 64-lane instructions, room for the XOR");
+  static_assert(KG == 1 || 64 % (BK * KG) == 0, "the K offset vectors are padded to a multiple of 64");
   static_assert(64 % BK == 0 && BM % RPI == 0 && BN % RPI == 0, "K-major blocks");
-  static_assert(BM % 64 == 0 || AKM, "M-major rows are whole DMA instructions");
-  static_assert(BN % 64 == 0 || BKM, "N-major rows are whole DMA instructions");
+  static_assert(VEC || BM % 64 == 0 || AKM, "M-major rows are whole DMA instructions");
+  static_assert(VEC || BN % 64 == 0 || BKM, "N-major rows are whole DMA instructions");
   static_assert(A_INSTR % NW == 0 && B_INSTR % NW == 0, "whole DMA instructions per wave");
   static_assert(BK % 8 == 0 && WM % 16 == 0 && WN % 16 == 0, "MFMA shape; an even number of k steps per slab");
   static_assert(LDS <= 160 * 1024, "LDS budget");
@@ -152,10 +163,14 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   typedef typename C::W W;
   typedef Mma<T> MM;
   typedef typename MM::acc Acc;
-  constexpr int BM = C::BM, BN = C::BN, BK = C::BK, NW = C::NW;
-  W* const lds = (W*)smem;
+  constexpr int BM = C::BM, BN = C::BN, BK = C::BK, NW = C::NW, KG = C::KG;
+  static_assert(!(SPLIT && KG > 1), "split K runs one K group");
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // K group kg (its own part of every stage) and the wave within it.
+  const int wave_all = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int kg = KG > 1 ? wave_all / NW : 0;
+  const int wave = KG > 1 ? wave_all % NW : wave_all;
+  W* const lds = (W*)smem + kg * C::STAGE;
   const int wm = (wave % C::WAVES_M) * C::WM;
   const int wn = (wave / C::WAVES_M) * C::WN;
   const int64_t M = p.M, N = p.N, K = p.K;
@@ -173,8 +188,9 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
     const int j = wave + NW * t;
     int r, kl;
     if constexpr (C::VEC && C::AKM) {
-      r = (j % (BM / 64)) * 64 + lane;
-      kl = C::V * (j / (BM / 64));
+      const int q = j * 64 + lane;
+      r = q % BM;
+      kl = C::V * (q / BM);
     } else if constexpr (C::VEC) {
       const int q = j * 64 + lane, k = q / (BM / C::V);
       r = ((q % (BM / C::V)) ^ (C::XQ * (k % C::XR))) * C::V;
@@ -195,8 +211,9 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
     const int j = wave + NW * t;
     int r, kl;
     if constexpr (C::VEC && C::BKM) {
-      r = (j % (BN / 64)) * 64 + lane;
-      kl = C::V * (j / (BN / 64));
+      const int q = j * 64 + lane;
+      r = q % BN;
+      kl = C::V * (q / BN);
     } else if constexpr (C::VEC) {
       const int q = j * 64 + lane, k = q / (BN / C::V);
       r = ((q % (BN / C::V)) ^ (C::XQ * (k % C::XR))) * C::V;
@@ -216,12 +233,14 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   auto a_dst = [&](int t) {
     const int j = wave + NW * t;
     if constexpr (C::VEC) return j * 64 * C::V;
-    return C::AKM ? j * (64 + C::PADB) : (j / (BM / 64)) * (BM + C::PADM) + (j % (BM / 64)) * 64;
+    else if constexpr (C::AKM) return j * (64 + C::PADB);
+    else return (j / (BM / 64)) * (BM + C::PADM) + (j % (BM / 64)) * 64;
   };
   auto b_dst = [&](int t) {
     const int j = wave + NW * t;
     if constexpr (C::VEC) return C::A_ELEMS + j * 64 * C::V;
-    return C::A_ELEMS + (C::BKM ? j * (64 + C::PADB) : (j / (BN / 64)) * (BN + C::PADM) + (j % (BN / 64)) * 64);
+    else if constexpr (C::BKM) return C::A_ELEMS + j * (64 + C::PADB);
+    else return C::A_ELEMS + (j / (BN / 64)) * (BN + C::PADM) + (j % (BN / 64)) * 64;
   };
 
   // K byte offsets of one slab from the padded 32-bit K scatters (the
@@ -236,7 +255,7 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   auto issue = [&](int st, const uint32_t* ka, const uint32_t* kb) {
 #pragma unroll
     for (int c = 0; c < C::PLANES; c++) {
-      W* stage = lds + st * C::STAGE + c * C::PLANE;
+      W* stage = lds + st * C::SSTRIDE + c * C::PLANE;
 #pragma unroll
       for (int t = 0; t < C::JA; t++) dma_load<C::DMAB>(rsA, stage + a_dst(t), a_row[t] + ka[t] + 4 * c);
 #pragma unroll
@@ -273,7 +292,7 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
     T b[FB];
   };
   auto load_frag = [&](int st, int kk, Frag& f) {
-    const W* As = lds + st * C::STAGE;
+    const W* As = lds + st * C::SSTRIDE;
     const W* Bs = As + C::A_ELEMS;
     if constexpr (!C::CPLX && C::PLANAR) {
       // f64 from its word planes.
@@ -399,37 +418,46 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   // read it), so the DMA of slab kt + 2 and the first fragments of slab
   // kt + 1 are issued under those MFMAs.
   constexpr int KQ = BK / KSTEP;
+  // K groups of one wave each read and fill only their own LDS parts: the
+  // wave's own waits order its DMA and its reads, and no barrier is needed
+  // until the groups meet in the epilogue.
+  constexpr bool SOLO = KG > 1 && NW == 1;
+  auto slab_barrier = [] {
+    if constexpr (!SOLO) __syncthreads();
+  };
   // p.kslabs counts slabs of 16 k (the planner's unit); a split's range is a
-  // whole number of this kernel's slabs (the launcher checks).
-  const int64_t nk_all = (K + BK - 1) / BK;
+  // whole number of this kernel's slabs (the launcher checks). With K groups
+  // an iteration covers KB = KG * BK k, group kg the BK of them at kg * BK.
+  constexpr int KB = BK * KG;
+  const int64_t nk_all = (K + KB - 1) / KB;
   const int64_t ks = SPLIT ? p.kslabs * 16 / BK : 0;
   const int64_t kt0 = SPLIT ? (int64_t)split * ks : 0;
   const int64_t nk_rest = nk_all - kt0;
   const int64_t nk = SPLIT ? (nk_rest < ks ? (nk_rest > 0 ? nk_rest : 0) : ks) : nk_all;
-  const int64_t kb0 = kt0 * BK;  // first k of the range
+  const int64_t kb0 = kt0 * BK + kg * BK;  // this group's first k of the range
   uint32_t a_ko[C::JA], b_ko[C::JB];
   if (nk > 0) {
     uint32_t a1[C::JA], b1[C::JB];
     load_koff(kb0, a_ko, b_ko);
-    if (nk > 1) load_koff(kb0 + BK, a1, b1);
+    if (nk > 1) load_koff(kb0 + KB, a1, b1);
     issue(0, a_ko, b_ko);
     if (nk > 1) issue(1, a1, b1);
   }
   __builtin_amdgcn_s_waitcnt(7 << 4);  // vmcnt(0) lgkmcnt(0)
-  __syncthreads();
+  slab_barrier();
   Frag f[C::PLR ? 2 : 1];
   if constexpr (!C::PLR) {
     for (int64_t kt = 0; kt < nk; kt++) {
       const bool more = kt + 2 < nk;
       const int st = (int)(kt & 1);
-      if (more) load_koff(kb0 + (kt + 2) * BK, a_ko, b_ko);
+      if (more) load_koff(kb0 + (kt + 2) * KB, a_ko, b_ko);
 #pragma unroll
       for (int q = 0; q < KQ; q++) {
         load_frag(st, KSTEP * q, f[0]);
         mma(f[0]);
       }
       __builtin_amdgcn_s_waitcnt(7 << 4);  // this wave's DMA of slab kt + 1 (and the offsets)
-      __syncthreads();
+      slab_barrier();
       if (more) issue(st, a_ko, b_ko);
     }
   } else {
@@ -437,14 +465,14 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   for (int64_t kt = 0; kt < nk; kt++) {
     const bool more = kt + 2 < nk;
     const int st = (int)(kt & 1);
-    if (more) load_koff(kb0 + (kt + 2) * BK, a_ko, b_ko);  // lands during the MFMAs
+    if (more) load_koff(kb0 + (kt + 2) * KB, a_ko, b_ko);  // lands during the MFMAs
 #pragma unroll
     for (int q = 0; q < KQ; q++) {
       if (q + 1 < KQ) {
         load_frag(st, KSTEP * (q + 1), f[(q + 1) & 1]);
       } else {
         __builtin_amdgcn_s_waitcnt(7 << 4);  // this wave's DMA of slab kt + 1, the offsets, its LDS reads
-        __syncthreads();
+        slab_barrier();
         if (more) issue(st, a_ko, b_ko);
         if (kt + 1 < nk) load_frag(st ^ 1, 0, f[(q + 1) & 1]);
       }
@@ -456,37 +484,46 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   }
 
   // Stage the tile chunk by chunk (Dre/Dim [EPI_ROWS][LDD]) and write it back.
+  // K groups store in turn, group 0 first and each later one adding its
+  // partial sums, so the order of the sum is fixed.
   constexpr int ER = C::EPI_ROWS;
   T* Dre = (T*)smem;
   T* Dim = Dre + ER * C::LDD;
+  auto put = [](T* dst, T v, bool add) { *dst = add ? *dst + v : v; };
+  if constexpr (SOLO) __syncthreads();  // every group is done with its stages
 #pragma unroll 1
   for (int c = 0; c < BM / ER; c++) {
     if (c > 0) __syncthreads();  // the previous chunk's write-back has read the staging rows
-    if (wm / ER == c) {
+#pragma unroll 1
+    for (int g = 0; g < KG; g++) {
+      if (g > 0) __syncthreads();  // group g - 1's values are in place
+      const bool add = g > 0;
+      if (kg == g && wm / ER == c) {
 #pragma unroll
-      for (int i = 0; i < C::MR; i++)
+        for (int i = 0; i < C::MR; i++)
 #pragma unroll
-        for (int j = 0; j < C::NI; j++)
+          for (int j = 0; j < C::NI; j++)
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int col = wn + 16 * j + lr;
-            if constexpr (C::METH == CPLX_1M) {
-              const int rr = 16 * i + MM::row(lk, q), row = wm - c * ER + rr / 2;
-              (rr & 1 ? Dim : Dre)[row * C::LDD + col] = acc[0][i][j][q];
-              continue;
+            for (int q = 0; q < 4; q++) {
+              const int col = wn + 16 * j + lr;
+              if constexpr (C::METH == CPLX_1M) {
+                const int rr = 16 * i + MM::row(lk, q), row = wm - c * ER + rr / 2;
+                put(&(rr & 1 ? Dim : Dre)[row * C::LDD + col], acc[0][i][j][q], add);
+                continue;
+              }
+              const int row = wm - c * ER + 16 * i + MM::row(lk, q);
+              if constexpr (C::METH == CPLX_3M) {
+                put(&Dre[row * C::LDD + col], acc[0][i][j][q] - acc[1][i][j][q], add);
+                put(&Dim[row * C::LDD + col], acc[2][i][j][q] - acc[0][i][j][q] - acc[1][i][j][q], add);
+              } else if constexpr (C::C3) {
+                put(&Dre[row * C::LDD + col], acc[0][i][j][q] - acc[1][i][j][q], add);
+                put(&Dim[row * C::LDD + col], acc[2][i][j][q], add);
+              } else {
+                put(&Dre[row * C::LDD + col], acc[0][i][j][q], add);
+                if constexpr (C::CPLX) put(&Dim[row * C::LDD + col], acc[1][i][j][q], add);
+              }
             }
-            const int row = wm - c * ER + 16 * i + MM::row(lk, q);
-            if constexpr (C::METH == CPLX_3M) {
-              Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
-              Dim[row * C::LDD + col] = acc[2][i][j][q] - acc[0][i][j][q] - acc[1][i][j][q];
-            } else if constexpr (C::C3) {
-              Dre[row * C::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
-              Dim[row * C::LDD + col] = acc[2][i][j][q];
-            } else {
-              Dre[row * C::LDD + col] = acc[0][i][j][q];
-              if constexpr (C::CPLX) Dim[row * C::LDD + col] = acc[1][i][j][q];
-            }
-          }
+      }
     }
     __syncthreads();
     if constexpr (SPLIT)
@@ -563,8 +600,13 @@ hipError_t launch_cfg(const ContractArgs& a, hipStream_t s) {
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H * (p.ksplit > 1 ? p.ksplit : 1);
   if (nwg == 0) return hipSuccess;
   if (nwg > max_grid(C::NT)) return hipErrorInvalidConfiguration;
-  if (p.ksplit > 1) hipLaunchKernelGGL((k_contract_dma<C, true>), dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
-  else hipLaunchKernelGGL((k_contract_dma<C, false>), dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
+  if constexpr (C::KG > 1) {
+    if (p.ksplit > 1) return hipErrorInvalidValue;  // K groups run one K range
+    hipLaunchKernelGGL((k_contract_dma<C, false>), dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
+  } else {
+    if (p.ksplit > 1) hipLaunchKernelGGL((k_contract_dma<C, true>), dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
+    else hipLaunchKernelGGL((k_contract_dma<C, false>), dim3((unsigned)nwg), dim3(C::NT), 0, s, p);
+  }
   return hipGetLastError();
 }
 
@@ -668,7 +710,58 @@ hipError_t grouped_by_method(const GroupItem* items, int n, int64_t total, int m
   return by_tile<T, CPLX, CPLX_NATIVE>(tile_id, Grouped<T, CPLX, CPLX_NATIVE>{items, n, total, grid, s});
 }
 
+// TILE_32x32K: a 32 x 32 tile on four K groups of one 32 x 32 wave each.
+template <typename T, bool CPLX, int METH, bool VEC, bool AKM, bool BKM>
+hipError_t launch_kgroup_cfg(const ContractArgs& a, hipStream_t s) {
+  return launch_cfg<DCfg<T, CPLX, 32, 32, 16, 32, 32, AKM, BKM, METH, true, 1, VEC, 4>>(a, s);
+}
+
+template <typename T, bool CPLX, int METH>
+hipError_t launch_kgroup_majors(const ContractArgs& a, hipStream_t s) {
+  typedef typename Store<T, CPLX>::S S;
+  if constexpr (sizeof(S) < 16) {
+    if (a.ab_vec) {
+      if (a.a_kmajor && a.b_kmajor) return launch_kgroup_cfg<T, CPLX, METH, true, true, true>(a, s);
+      // 16 chunks of 2 elements per k of a 32-row M/N-major part (f64, c32).
+      if constexpr (sizeof(S) == 8) {
+        if (a.a_kmajor) return launch_kgroup_cfg<T, CPLX, METH, true, true, false>(a, s);
+        if (a.b_kmajor) return launch_kgroup_cfg<T, CPLX, METH, true, false, true>(a, s);
+        return launch_kgroup_cfg<T, CPLX, METH, true, false, false>(a, s);
+      }
+      return hipErrorInvalidValue;
+    }
+  }
+  if (a.a_kmajor && a.b_kmajor) return launch_kgroup_cfg<T, CPLX, METH, false, true, true>(a, s);
+  return hipErrorInvalidValue;
+}
+
+template <typename T, bool CPLX>
+hipError_t kgroup_by_method(const ContractArgs& a, int method, hipStream_t s) {
+  if constexpr (CPLX) {
+    if (method == CPLX_3M) return launch_kgroup_majors<T, true, CPLX_3M>(a, s);
+    if (method == CPLX_1M) return launch_kgroup_majors<T, true, CPLX_1M>(a, s);
+  }
+  return launch_kgroup_majors<T, CPLX, CPLX_NATIVE>(a, s);
+}
+
 }  // namespace
+
+bool kgroup_supported(bool off32, bool ab_vec, bool a_kmajor, bool b_kmajor, DType t) {
+  if (!off32) return false;
+  if (ab_vec && t != DType::C64) return (a_kmajor && b_kmajor) || t == DType::F64 || t == DType::C32;
+  return a_kmajor && b_kmajor;
+}
+
+hipError_t launch_contract_kgroup(const ContractArgs& a, DType t, int method, hipStream_t s) {
+  if (a.ksplit > 1 || !kgroup_supported(a.off32, a.ab_vec, a.a_kmajor, a.b_kmajor, t)) return hipErrorInvalidValue;
+  switch (t) {
+    case DType::C64: return kgroup_by_method<double, true>(a, method, s);
+    case DType::C32: return kgroup_by_method<float, true>(a, method, s);
+    case DType::F64: return kgroup_by_method<double, false>(a, method, s);
+    case DType::F32: return kgroup_by_method<float, false>(a, method, s);
+  }
+  return hipErrorInvalidValue;
+}
 
 // Every dtype and complex schedule with 32-bit gather offsets. On the complex
 // benchmark the c64 64x64 DMA kernel ran 2.41 ms against 2.98 ms for the

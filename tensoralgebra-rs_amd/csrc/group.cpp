@@ -168,11 +168,12 @@ void run_group(Group& g, Executor& ex) {
   ex.finish(s, !ex.async.load());
 }
 
-// Storage tiles a group can run: 128x64 is not offered for c64, and the
-// DMA-only 128x128 tile needs 32-bit gather offsets in every item.
+// Storage tiles a group can run: 128x64 is not offered for c64, the
+// DMA-only 128x128 tile needs 32-bit gather offsets in every item, and the
+// K-group 32x32 tile has no grouped form.
 bool group_tile_ok(const Group& g, int id) {
   if (id == TILE_128x128) return g.all_off32 && dma_supported(true, g.dtype, g.method, id);
-  return tile_cfg(id) != nullptr && id < 10 && !(id == TILE_128x64 && g.dtype == DType::C64);
+  return tile_cfg(id) != nullptr && id < 10 && id != TILE_32x32K && !(id == TILE_128x64 && g.dtype == DType::C64);
 }
 
 Group* group_of(tprims_group h) {

@@ -21,6 +21,7 @@ enum TileId : int {
   TILE_32x32 = 2,  // 1 wave of 32x32
   TILE_128x64 = 3, // 4 waves of 64x32
   TILE_128x128 = 4, // LDS-DMA kernel only (16 waves of 32x32, or 8 of 64x32: by dtype, schedule, majors)
+  TILE_32x32K = 5,  // LDS-DMA kernel only: 4 waves of 32x32, each over every fourth K slab
 };
 // Tile ids of the split-BF16 kernels (16x16x32 bf16 MFMA family).
 enum SplitTileId : int {
@@ -44,6 +45,11 @@ hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_
 // launch_contract picks whenever it applies, split K included.
 bool dma_supported(bool off32, DType t, int method, int tile_id);
 hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int tile_id, hipStream_t s);
+// The 32x32 tile on four K groups (TILE_32x32K): 32-bit gather offsets, one
+// K range (no split K), and K-major A and B unless both gather in 16-byte
+// runs of f64 or c32 elements (any majors) or f32 elements (K-major).
+bool kgroup_supported(bool off32, bool ab_vec, bool a_kmajor, bool b_kmajor, DType t);
+hipError_t launch_contract_kgroup(const ContractArgs& a, DType t, int method, hipStream_t s);
 // Split-BF16 for f32/c32 storage (nterms 2 or 3): A and B are split once into
 // bf16 fragment panels (device scratch of split_panel_bytes), then multiplied.
 size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id);

@@ -292,13 +292,19 @@ void configure(Plan* plan, const PlanConfig& cfg) {
   // split at least 6 slabs, at most 16 (f64 576^3, 81 tiles: 3 splits 25.4 us
   // per call, 4 splits 30.5 (a second round on 68 CUs), 6 splits 27.6).
   // cfg.split_k = 1 keeps one K range, n > 1 asks for n splits (at most one
-  // per slab).
+  // per slab). For real dtypes with K <= 1024, where 32x32 tiles alone would
+  // give at least 192 workgroups, the K-group 32x32 tile replaces split K
+  // (chosen below, once the gather form is known): f64 24^6 20.1 against 21.0
+  // us per call, f32 640 x 1024 x 640 20.5 against 23.3; with longer K the
+  // 64x64 tile's better MFMA rate per slab wins (f64 520 x 2048 x 520: 45.6
+  // split against 68), and so does split K for c32 (24^6: 29.6 against 31.2).
   p.ksplit = 1;
   p.kslabs = 0;
   auto tiles_of = [&](int id) {
     const TileCfg* tc = tile_cfg(id);
     return ((p.M + tc->bm - 1) / tc->bm) * ((p.N + tc->bn - 1) / tc->bn) * p.H;
   };
+  bool kgroup_candidate = false;
   if (cfg.split_k < 0 || cfg.split_k > 64) fail(ERR_INVALID_ARGUMENT, "split_k must be in [0, 64]");
   if (p.strategy == 0 && cfg.precision == PREC_STORAGE && cfg.split_k != 1) {
     const TileCfg* tc = tile_cfg(p.tile);
@@ -306,10 +312,13 @@ void configure(Plan* plan, const PlanConfig& cfg) {
     const int64_t nk = (p.K + tc->bk - 1) / tc->bk;
     int64_t s = 1;
     if (cfg.split_k > 1) s = std::min<int64_t>(cfg.split_k, nk);
-    else if (tiles < 128 && nk >= 16) s = std::min<int64_t>(std::min<int64_t>(256 / tiles, nk / 6), 16);
+    else if (tiles < 128 && nk >= 16 && p.tile != TILE_32x32K)
+      s = std::min<int64_t>(std::min<int64_t>(256 / tiles, nk / 6), 16);
     if (s >= 2) {
       p.ksplit = (int)s;
       p.kslabs = (nk + s - 1) / s;
+      kgroup_candidate = cfg.split_k == 0 && cfg.tile == 0 && !is_complex(pr.dtype) && p.K <= 1024 &&
+                         tiles_of(TILE_32x32K) >= 192;
     }
   }
   p.ak32.clear();
@@ -358,6 +367,11 @@ void configure(Plan* plan, const PlanConfig& cfg) {
     };
     p.a_vec16 = operand(p.a_kmajor, p.am, p.M, p.ak, p.ha);
     p.ab_vec = p.a_vec16 && operand(p.b_kmajor, p.bn, p.N, p.bk, p.hb);
+  }
+  if (kgroup_candidate && kgroup_supported(p.off32, p.ab_vec, p.a_kmajor, p.b_kmajor, pr.dtype)) {
+    p.tile = TILE_32x32K;
+    p.ksplit = 1;
+    p.kslabs = 0;
   }
   // Split-BF16 that splits A inside the GEMM runs its 128x128 tile (two
   // workgroups per CU, whose splitting and MFMAs overlap) rather than
@@ -560,10 +574,15 @@ std::string Plan::kernel_name() const {
   }
   // The DMA-only tile runs as 64x64 on the register-staged kernel without 32-bit offsets.
   const bool fallback = tile == TILE_128x128 && !dma_supported(off32, prob.dtype, cfg.complex_method, tile);
+  // The K-group tile runs without split K where its gathers fit, else as the
+  // register-staged 32x32 tile.
+  const bool kgroup = tile == TILE_32x32K && ksplit <= 1 && kgroup_supported(off32, ab_vec, a_kmajor, b_kmajor, prob.dtype);
   const TileCfg* t = tile_cfg(fallback ? TILE_64x64 : tile);
   if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(cfg.precision == PREC_FP8 ? 128 : t->bk);
-  if (cfg.precision == PREC_STORAGE && !fallback && dma_supported(off32, prob.dtype, cfg.complex_method, tile))
+  if (cfg.precision == PREC_STORAGE &&
+      ((!fallback && dma_supported(off32, prob.dtype, cfg.complex_method, tile)) || kgroup))
     s += ab_vec ? ".dma16" : ".dma";
+  if (kgroup) s += ".kgroup4";
   if (ksplit > 1) s += ".splitk" + std::to_string(ksplit);
   // Split-BF16 with an f32 A gathered along M in 16-byte runs: A is split
   // inside the GEMM (for a 16-byte aligned A, checked per call).

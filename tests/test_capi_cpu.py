@@ -315,14 +315,19 @@ def test_planner_picks_elementwise_for_all_batch_products():
 
 
 def test_split_k_choice_and_override():
-    """Few output tiles with a long K split K (planner's choice); split_k = 1
-    keeps one K range, n > 1 asks for n; out-of-range values are refused."""
+    """Few 64x64 output tiles with a long K: the planner takes the K-group
+    32x32 tile where 32x32 tiles give at least 192 workgroups (the f64 24^6
+    contraction: 324) and its gathers allow, else splits K (the f32 form of
+    the same contraction, whose A gathers along M); split_k = 1 keeps one K
+    range, n > 1 asks for n; out-of-range values are refused."""
     import tprims_amd as tp
     from harness import T, make_plan
     e = 24
     sp = [1, e, e * e, e ** 3]
     args = (T([e] * 4, sp, [0, 4, 1, 5]), T([e] * 4, sp, [5, 2, 4, 3]), T([e] * 4, sp, [0, 1, 2, 3]))
-    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.64x64x16.dma16.splitk3"
+    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.32x32x16.dma16.kgroup4"
+    assert make_plan(tp.DType.F64, *args).report()["tile"] == 5
+    assert make_plan(tp.DType.F32, *args).report()["kernel"] == "f32.mfma_f32.64x64x16.dma16.splitk3"
     assert ".splitk" not in make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=1)).report()["kernel"]
     assert make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=5)).report()["kernel"].endswith(".splitk5")
     with pytest.raises(tp.TprimsError):

@@ -59,24 +59,28 @@ def _report(plan):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", ["default", "one_k_range", "host_buffers"])
+@pytest.mark.parametrize("variant", ["default", "one_k_range", "split_k3", "host_buffers"])
 def test_f64_24_contraction_through_the_tapp_abi(gpu, variant):
     """D[a,b,c,d] = sum_{e,f} A[a,e,b,f] B[f,c,e,d], every extent 24,
     column-major, U[-1,1] seeded, alpha 1, beta 0: the whole D against the
-    oracle at the reference's f64 budget (driver/tests/correctness.rs:77-88)."""
+    oracle at the reference's f64 budget (driver/tests/correctness.rs:77-88).
+    The planner's choice (the K-group 32x32 tile on 16-byte runs), one K range
+    on the 64x64 tile, and three K ranges with the split-K reduction."""
     import torch
 
     e = 24
     a = T([e] * 4, tp.col_major([e] * 4), [A_, E_, B_, F_])
     b = T([e] * 4, tp.col_major([e] * 4), [F_, C_, E_, D_])
     d = T([e] * 4, tp.col_major([e] * 4), [A_, B_, C_, D_])
-    cfg = tp.PlanConfig(split_k=1) if variant == "one_k_range" else None
+    cfg = {"one_k_range": tp.PlanConfig(split_k=1), "split_k3": tp.PlanConfig(split_k=3)}.get(variant)
     plan, handle = _tapp_product(tp.DType.F64, [a, b, d, d], cfg=cfg)
     kern = _report(plan).kernel.decode()
     if variant == "one_k_range":
-        assert ".splitk" not in kern, kern
+        assert ".splitk" not in kern and ".kgroup" not in kern, kern
+    elif variant == "split_k3":
+        assert ".dma16.splitk3" in kern, kern
     else:
-        assert ".dma16.splitk" in kern, kern  # the planner's choice: 16-byte runs, K split
+        assert ".dma16.kgroup4" in kern, kern  # the planner's choice: 16-byte runs, four K groups
     rng = np.random.default_rng(24)
     av, bv = rng.uniform(-1, 1, e ** 4), rng.uniform(-1, 1, e ** 4)
     dv = np.full(e ** 4, np.nan)  # beta = 0: D is written, never read
