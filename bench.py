@@ -241,6 +241,8 @@ def config_precision(torch, tp, np, ex, stream, steps, shape):
                                       specs(d_dims), tp.Labels(a_lab, b_lab, d_lab, d_lab))
         plan = tp.Plan(prob, tp.PlanConfig(precision=prec))
         run = lambda: plan.execute_raw(1.0, A.data_ptr(), B.data_ptr(), 0.0, None, D.data_ptr(), ex)
+        D.fill_(float("nan"))  # every precision writes all of D itself (beta = 0)
+        torch.cuda.synchronize()
         ms = timed(torch, run, stream, max(3, steps // 2), 5)
         Dh = D.cpu().numpy().reshape(N, M)
         got = np.array([Dh[j, i] for i, j in zip(si, sj)], dtype=np.float64)

@@ -175,6 +175,59 @@ __device__ inline void write_back(const ContractArgs& p, int64_t h, int64_t m0, 
   const T ber = (T)p.beta[0], bei = (T)p.beta[1];
   const bool rc = p.read_c, cc = p.conj_c, cd = p.conj_d;
   const int64_t M = p.M, N = p.N, R = p.R;
+  // Real dtypes whose D (and C, when read) is contiguous along the staged
+  // axis, with affine offsets, no broadcast copies and 16-byte aligned runs:
+  // V consecutive elements per lane as one 16-byte load / store, with
+  // (alpha, beta) applied in registers. A ragged end of the contiguous axis
+  // falls back to single elements.
+  constexpr int V = 16 / (int)sizeof(T);
+  if constexpr (!CPLX && BM % V == 0 && BN % V == 0 && BM * BN % (V * NT) == 0) {
+    const bool mm = p.d_mmajor;
+    constexpr int RM = BM / V, RN = BN / V;  // vectors per row / column of the tile
+    if (p.mn_affine && p.dr == nullptr) {
+      const int64_t ds_c = mm ? p.cd_stride[2] : p.cd_stride[3];  // contiguous axis
+      const int64_t ds_o = mm ? p.cd_stride[3] : p.cd_stride[2];  // other axis
+      const int64_t cs_c = mm ? p.cd_stride[0] : p.cd_stride[1];
+      const int64_t cs_o = mm ? p.cd_stride[1] : p.cd_stride[0];
+      const bool ok = ds_c == 1 && ds_o % V == 0 && (uintptr_t)D % 16 == 0 &&
+                      (!rc || (cs_c == 1 && cs_o % V == 0 && (uintptr_t)Cp % 16 == 0));
+      if (ok) {
+        typedef T vec __attribute__((ext_vector_type(16 / sizeof(T))));
+        constexpr int EPV = BM * BN / (V * NT);
+#pragma unroll 4
+        for (int t = 0; t < EPV; t++) {
+          const int e = tid + t * NT;
+          const int ml = mm ? V * (e % RM) : e / RN, nl = mm ? e / RM : V * (e % RN);
+          const int64_t m = m0 + ml, n = n0 + nl;
+          if (m >= M || n >= N) continue;
+          const int64_t lim = mm ? M - m : N - n;  // elements left along the contiguous axis
+          T x[V];
+#pragma unroll
+          for (int q = 0; q < V; q++) x[q] = (T)(mm ? Dre[(ml + q) * LDD + nl] : Dre[ml * LDD + nl + q]);
+          const int64_t off = m * p.cd_stride[2] + n * p.cd_stride[3];
+          const int64_t coff = m * p.cd_stride[0] + n * p.cd_stride[1];
+          if (lim >= V) {
+            vec y;
+#pragma unroll
+            for (int q = 0; q < V; q++) y[q] = alr * x[q];
+            if (rc) {
+              const vec c = *(const vec*)(Cp + coff);
+#pragma unroll
+              for (int q = 0; q < V; q++) y[q] += ber * c[q];
+            }
+            *(vec*)(D + off) = y;
+          } else {
+            for (int q = 0; q < lim; q++) {
+              T y = alr * x[q];
+              if (rc) y += ber * Cp[coff + q];
+              D[off + q] = y;
+            }
+          }
+        }
+        return;
+      }
+    }
+  }
   constexpr int EPI = BM * BN / NT;
 #pragma unroll 4
   for (int t = 0; t < EPI; t++) {

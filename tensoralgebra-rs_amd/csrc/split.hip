@@ -490,9 +490,49 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
   }
   __syncthreads();  // every wave is done with the stages before they become the staging tile
 
+  const int lr = lane & 15, lk = lane >> 4;
+  if constexpr (!G::CPLX && (size_t)BN * (BM + 4) * 4 <= G::LDS && BM * BN % (4 * NT) == 0) {
+    // D contiguous along M with affine offsets, no broadcast copies, 16-byte
+    // aligned columns: the tile is staged column by column (pitch BM + 4, so
+    // each 16-lane group's 16-byte stores hit distinct banks), a lane's four
+    // consecutive rows of an accumulator block as one 16-byte store, and
+    // written back as 16-byte vectors.
+    float* __restrict__ Dg = (float*)p.D + p.hd[h];
+    const float* __restrict__ Cg = (const float*)p.C + p.hc[h];
+    const bool ok = p.d_mmajor && p.mn_affine && p.dr == nullptr && p.cd_stride[2] == 1 && p.cd_stride[3] % 4 == 0 &&
+                    (uintptr_t)Dg % 16 == 0 &&
+                    (!p.read_c || (p.cd_stride[0] == 1 && p.cd_stride[1] % 4 == 0 && (uintptr_t)Cg % 16 == 0));
+    if (ok) {
+      constexpr int LDM = BM + 4;
+      float* Dt = (float*)smem;
+#pragma unroll
+      for (int i = 0; i < G::MI; i++)
+#pragma unroll
+        for (int j = 0; j < G::NI; j++) *(v4f*)&Dt[(wn + 16 * j + lr) * LDM + wm + 16 * i + 4 * lk] = acc[0][i][j];
+      __syncthreads();
+      const float alr = (float)p.alpha[0], ber = (float)p.beta[0];
+      const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+#pragma unroll 4
+      for (int t = 0; t < BM * BN / (4 * NT); t++) {
+        const int e = tid + t * NT;
+        const int ml = 4 * (e % (BM / 4)), nl = e / (BM / 4);
+        const int64_t m = m0 + ml, n = n0 + nl;
+        if (m >= p.M || n >= p.N) continue;
+        const v4f x = *(const v4f*)&Dt[nl * LDM + ml];
+        const int64_t off = m + n * p.cd_stride[3], coff = m + n * p.cd_stride[1];
+        if (p.M - m >= 4) {
+          v4f y = alr * x;
+          if (p.read_c) y += ber * *(const v4f*)(Cg + coff);
+          *(v4f*)(Dg + off) = y;
+        } else {
+          for (int q = 0; q < p.M - m; q++) Dg[off + q] = alr * x[q] + (p.read_c ? ber * Cg[coff + q] : 0.0f);
+        }
+      }
+      return;
+    }
+  }
   float* Dre = (float*)smem;
   float* Dim = Dre + BM * G::LDD;
-  const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int i = 0; i < G::MI; i++)
 #pragma unroll

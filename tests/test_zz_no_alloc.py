@@ -29,7 +29,8 @@ def _call(fn, plan, ex, alpha, a, b, beta, c, d, dtype):
 
 def _big_case():
     """The f64 24^6 contraction D[a,b,c,d] = sum_{e,f} A[a,e,b,f] B[f,c,e,d]
-    (split K: scatter vectors and partial tiles both in the scratch)."""
+    with three K ranges asked for (split K: scatter vectors and partial tiles
+    both in the scratch)."""
     e = 24
     a = T([e] * 4, tp.col_major([e] * 4), [1, 2, 3, 4])
     b = T([e] * 4, tp.col_major([e] * 4), [4, 5, 2, 6])
@@ -62,7 +63,7 @@ def test_noalloc_matches_execute_product_on_two_calls(gpu, asynchronous):
     jobs = []
     for k, case in enumerate((_big_case, _small_case)):  # the second call is smaller
         dtype, a, b, d, alpha, beta = case()
-        plan = make_plan(dtype, a, b, d, d)
+        plan = make_plan(dtype, a, b, d, d, tp.PlanConfig(split_k=3) if k == 0 else None)
         if k == 0:
             assert ".splitk" in plan.report()["kernel"]
         av, bv, cv, (ta, tb, tc) = _device_operands(dtype, a, b, d, 10 * k + 1)
