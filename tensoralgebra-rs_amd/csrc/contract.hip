@@ -438,7 +438,9 @@ __device__ inline void contract_tile(const ContractArgs& p, int64_t h, int tm, i
     if (p.ksplit > 1)
       dev::write_partial<T, C::CPLX, ER, BN, NT, C::LDD>(p, split, h, m0 + c * ER, n0, Dre, Dim);
     else
-      dev::write_back<T, T, C::CPLX, ER, BN, NT, C::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
+      // Element-wise write-back: the vector path costs these kernels registers
+      // (occupancy of the grouped 32x32 bodies 2 -> 1 waves per SIMD).
+      dev::write_back<T, T, C::CPLX, ER, BN, NT, C::LDD, false>(p, h, m0 + c * ER, n0, Dre, Dim);
   }
 }
 

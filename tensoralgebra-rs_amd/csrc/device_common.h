@@ -163,7 +163,8 @@ __device__ inline void store_element(const ContractArgs& p, int64_t h, int64_t m
 // Writes a staged BM x BN tile (Dre/Dim, row pitch LDD, in LDS) to D along
 // D's contiguous axis: alpha scaling, beta * op_C(C) only when beta != 0 (C is
 // never read otherwise), op_D last, and the broadcast copies of an adjoint.
-template <typename T, typename TS, bool CPLX, int BM, int BN, int NT, int LDD>
+// VEC: allow the 16-byte vector path below (it costs registers).
+template <typename T, typename TS, bool CPLX, int BM, int BN, int NT, int LDD, bool VEC = true>
 __device__ inline void write_back(const ContractArgs& p, int64_t h, int64_t m0, int64_t n0, const TS* Dre,
                                   const TS* Dim) {
   typedef Store<T, CPLX> St;
@@ -181,7 +182,7 @@ __device__ inline void write_back(const ContractArgs& p, int64_t h, int64_t m0, 
   // (alpha, beta) applied in registers. A ragged end of the contiguous axis
   // falls back to single elements.
   constexpr int V = 16 / (int)sizeof(T);
-  if constexpr (!CPLX && BM % V == 0 && BN % V == 0 && BM * BN % (V * NT) == 0) {
+  if constexpr (VEC && !CPLX && BM % V == 0 && BN % V == 0 && BM * BN % (V * NT) == 0) {
     const bool mm = p.d_mmajor;
     constexpr int RM = BM / V, RN = BN / V;  // vectors per row / column of the tile
     if (p.mn_affine && p.dr == nullptr) {
@@ -194,7 +195,9 @@ __device__ inline void write_back(const ContractArgs& p, int64_t h, int64_t m0, 
       if (ok) {
         typedef T vec __attribute__((ext_vector_type(16 / sizeof(T))));
         constexpr int EPV = BM * BN / (V * NT);
-#pragma unroll 4
+        // Not unrolled: unrolling raised the register count of the f32
+        // 128x128 contraction kernel from 68 to 128 (half the occupancy).
+#pragma unroll 1
         for (int t = 0; t < EPV; t++) {
           const int e = tid + t * NT;
           const int ml = mm ? V * (e % RM) : e / RN, nl = mm ? e / RM : V * (e % RN);

@@ -44,6 +44,7 @@ namespace {
 using dev::Cx;
 using dev::Mma;
 using dev::Store;
+using dev::v4f;
 
 // One LDS-DMA instruction: every lane's ES bytes at byte offset `off` of the
 // resource go to LDS at dst + 16 lane (ES 16) or dst + 4 lane (ES 4). The
@@ -127,7 +128,14 @@ struct DCfg {
   // otherwise one wave row (WM rows) at a time.
   static constexpr size_t EPI_ROW_BYTES = (size_t)(CPLX ? 2 : 1) * LDD * sizeof(T);
   static constexpr int EPI_ROWS = (size_t)BM * EPI_ROW_BYTES <= LDS_MAIN ? BM : WM;
-  static constexpr size_t LDS_EPI = (size_t)EPI_ROWS * EPI_ROW_BYTES;
+  static constexpr size_t LDS_EPI0 = (size_t)EPI_ROWS * EPI_ROW_BYTES;
+  // f32 (real): a lane's four accumulator rows are consecutive, so an output
+  // contiguous along M can be staged column-major, one wave column (WN
+  // columns, pitch BM + 4) at a time, with 16-byte LDS stores and reads.
+  static constexpr bool COLS = !CPLX && sizeof(T) == 4;
+  static constexpr int LDM = BM + 4;
+  static constexpr size_t LDS_COL = COLS ? (size_t)WN * LDM * 4 : 0;
+  static constexpr size_t LDS_EPI = LDS_EPI0 > LDS_COL ? LDS_EPI0 : LDS_COL;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
   static_assert(DMAB == 4 || DMAB == 16, "LDS DMA moves 4 or 16 bytes per lane");
   // Run layout: whole 64-lane instructions; an M/N-major part needs a
@@ -157,7 +165,10 @@ struct DCfg {
 // through smem (C::LDS bytes): over all of K, written back to D; or, with
 // split K, over K range `split` (slabs [split * kslabs, + kslabs)), written
 // unscaled to the split-K workspace.
-template <class C, bool SPLIT = false>
+// FAST: the vector epilogues (column-major staging for f32, 16-byte stores in
+// the write-back); the grouped body, which inlines four gather orientations,
+// keeps the element-wise write-back to hold its register count.
+template <class C, bool SPLIT = false, bool FAST = true>
 __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn, char* smem, int split = 0) {
   typedef typename C::Tv T;
   typedef typename C::W W;
@@ -483,6 +494,59 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
   }
   }
 
+  if constexpr (C::COLS && !SPLIT && FAST) {
+    // f32 output contiguous along M (affine offsets, no broadcast copies,
+    // 16-byte aligned columns): stage one wave column of the tile at a time,
+    // column-major, and write it back as 16-byte vectors (alpha and beta * C
+    // in registers). K groups add in turn, in group order.
+    float* __restrict__ Dg = (float*)p.D + p.hd[h];
+    const float* __restrict__ Cg = (const float*)p.C + p.hc[h];
+    const bool col_ok = p.d_mmajor && p.mn_affine && p.dr == nullptr && p.cd_stride[2] == 1 &&
+                        p.cd_stride[3] % 4 == 0 && (uintptr_t)Dg % 16 == 0 &&
+                        (!p.read_c || (p.cd_stride[0] == 1 && p.cd_stride[1] % 4 == 0 && (uintptr_t)Cg % 16 == 0));
+    if (col_ok) {
+      constexpr int LDM = C::LDM, WN = C::WN, NTH = C::NT;
+      float* Dt = (float*)smem;
+      const float alr = (float)p.alpha[0], ber = (float)p.beta[0];
+      if constexpr (SOLO) __syncthreads();  // every group is done with its stages
+#pragma unroll 1
+      for (int c = 0; c < BN / WN; c++) {
+        __syncthreads();  // the stages, or the previous column's staging, are free
+#pragma unroll 1
+        for (int g = 0; g < KG; g++) {
+          if (g > 0) __syncthreads();
+          if (kg == g && wn == c * WN) {
+#pragma unroll
+            for (int i = 0; i < C::MR; i++)
+#pragma unroll
+              for (int j = 0; j < C::NI; j++) {
+                v4f* dst = (v4f*)&Dt[(16 * j + lr) * LDM + wm + 16 * i + 4 * lk];
+                const v4f v = __builtin_bit_cast(v4f, acc[0][i][j]);
+                *dst = g > 0 ? *dst + v : v;
+              }
+          }
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int e = threadIdx.x; e < BM / 4 * WN; e += NTH) {
+          const int ml = 4 * (e % (BM / 4)), nl = e / (BM / 4);
+          const int64_t m = m0 + ml, n = n0 + c * WN + nl;
+          if (m >= M || n >= N) continue;
+          const v4f x = *(const v4f*)&Dt[nl * LDM + ml];
+          const int64_t off = m + n * p.cd_stride[3], coff = m + n * p.cd_stride[1];
+          if (M - m >= 4) {
+            v4f y = alr * x;
+            if (p.read_c) y += ber * *(const v4f*)(Cg + coff);
+            *(v4f*)(Dg + off) = y;
+          } else {
+            for (int q = 0; q < M - m; q++) Dg[off + q] = alr * x[q] + (p.read_c ? ber * Cg[coff + q] : 0.0f);
+          }
+        }
+      }
+      return;
+    }
+  }
+
   // Stage the tile chunk by chunk (Dre/Dim [EPI_ROWS][LDD]) and write it back.
   // K groups store in turn, group 0 first and each later one adding its
   // partial sums, so the order of the sum is fixed.
@@ -529,7 +593,7 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
     if constexpr (SPLIT)
       dev::write_partial<T, C::CPLX, ER, BN, C::NT, C::LDD>(p, split, h, m0 + c * ER, n0, Dre, Dim);
     else
-      dev::write_back<T, T, C::CPLX, ER, BN, C::NT, C::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
+      dev::write_back<T, T, C::CPLX, ER, BN, C::NT, C::LDD, FAST>(p, h, m0 + c * ER, n0, Dre, Dim);
   }
 }
 
@@ -574,6 +638,8 @@ __global__ __launch_bounds__(F::NT, F::MINW) void k_grouped_dma(const GroupItem*
       if (items[mid].tile_begin <= t) lo = mid;
       else hi = mid - 1;
     }
+    // The item is the workgroup's: its fields load once into scalar registers.
+    lo = __builtin_amdgcn_readfirstlane(lo);
     const ContractArgs& p = items[lo].args;
     const int64_t local = t - items[lo].tile_begin;
     const int64_t per_h = (int64_t)p.tiles_m * p.tiles_n;
@@ -581,11 +647,11 @@ __global__ __launch_bounds__(F::NT, F::MINW) void k_grouped_dma(const GroupItem*
     const int r = (int)(local - h * per_h);
     const int tm = r % p.tiles_m, tn = r / p.tiles_m;
     if (p.a_kmajor) {
-      if (p.b_kmajor) dma_tile<typename F::KK>(p, h, tm, tn, smem);
-      else dma_tile<typename F::KN>(p, h, tm, tn, smem);
+      if (p.b_kmajor) dma_tile<typename F::KK, false, false>(p, h, tm, tn, smem);
+      else dma_tile<typename F::KN, false, false>(p, h, tm, tn, smem);
     } else {
-      if (p.b_kmajor) dma_tile<typename F::MK>(p, h, tm, tn, smem);
-      else dma_tile<typename F::MN>(p, h, tm, tn, smem);
+      if (p.b_kmajor) dma_tile<typename F::MK, false, false>(p, h, tm, tn, smem);
+      else dma_tile<typename F::MN, false, false>(p, h, tm, tn, smem);
     }
     __syncthreads();  // the next tile's DMA overwrites this tile's staging
   }
@@ -679,8 +745,23 @@ struct Grouped {
   int64_t total;
   int grid;
   hipStream_t s;
+  // The grouped body holds the persistent loop's state and all four gather
+  // orientations, so it runs its 128x128 tiles one workgroup per CU with no
+  // register bound (f64: no spills, where the single launch's two workgroups
+  // per CU spill outside the K loop). The c64 and c32 1m 128x128 bodies still
+  // spill inside the K loop within a 16-wave workgroup's 128 registers, so
+  // groups do not offer that tile for them and they are not built.
   template <int BM, int BN, int WM, int WN, bool PLR, int MINW>
   hipError_t go() const {
+    constexpr bool big = BM == 128 && BN == 128;
+    if constexpr (big && CPLX && (sizeof(T) == 8 || METH == CPLX_1M)) {
+      return hipErrorInvalidValue;
+    } else {
+      return launch<BM, BN, WM, WN, PLR, big ? 1 : MINW>();
+    }
+  }
+  template <int BM, int BN, int WM, int WN, bool PLR, int MINW>
+  hipError_t launch() const {
     typedef DFamily<T, CPLX, BM, BN, 16, WM, WN, METH, PLR, MINW> F;
     static_assert(F::LDS <= 160 * 1024, "LDS budget");
     if (total == 0) return hipSuccess;

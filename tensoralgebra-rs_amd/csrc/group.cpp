@@ -169,10 +169,14 @@ void run_group(Group& g, Executor& ex) {
 }
 
 // Storage tiles a group can run: 128x64 is not offered for c64, the
-// DMA-only 128x128 tile needs 32-bit gather offsets in every item, and the
-// K-group 32x32 tile has no grouped form.
+// DMA-only 128x128 tile needs 32-bit gather offsets in every item and is not
+// offered for c64 (any schedule) or the c32 1m schedule, whose grouped bodies
+// do not fit the registers of a 16-wave workgroup (spills inside the K loop),
+// and the K-group 32x32 tile has no grouped form.
 bool group_tile_ok(const Group& g, int id) {
-  if (id == TILE_128x128) return g.all_off32 && dma_supported(true, g.dtype, g.method, id);
+  if (id == TILE_128x128)
+    return g.all_off32 && dma_supported(true, g.dtype, g.method, id) && g.dtype != DType::C64 &&
+           !(g.dtype == DType::C32 && g.method == CPLX_1M);
   return tile_cfg(id) != nullptr && id < 10 && id != TILE_32x32K && !(id == TILE_128x64 && g.dtype == DType::C64);
 }
 

@@ -109,13 +109,13 @@ def test_group_autotune_keeps_results(gpu):
     plans, al, be, A, B, C, D, want = build(dtype, mps_items(dtype, 12, 9), gpu)
     g = tp.Group(plans, al, A, B, be, C, D)
     tile, ms = g.autotune(reps=2)
-    assert tile in (1, 2, 4) and ms > 0
+    assert tile in (1, 2) and ms > 0  # c64 groups are not offered the 128x128 tile
     for d in D:
         d.zero_()
     g.execute()
     torch.cuda.synchronize()
     check_all(dtype, D, want, "autotuned")
-    for t in (1, 2, 4):
+    for t in (1, 2):
         g.set_tile(t)
         g.execute()
         torch.cuda.synchronize()
@@ -128,7 +128,8 @@ def test_group_autotune_keeps_results(gpu):
                          ids=lambda m: m.name)
 def test_group_on_every_dma_tile(gpu, dtype, method):
     """The grouped kernel on the LDS-DMA tiles (64x64, 128x64 where offered,
-    128x128) for every complex schedule, against the oracle."""
+    128x128 where offered: not for c64 or the c32 1m schedule, which set_tile
+    refuses) for every complex schedule, against the oracle."""
     import torch
 
     if not dtype.is_complex and method != tp.ComplexMethod.NATIVE:
@@ -140,6 +141,10 @@ def test_group_on_every_dma_tile(gpu, dtype, method):
     g = tp.Group(plans, al, A, B, be, C, D)
     for t in (1, 3, 4):
         if t == 3 and dtype == tp.DType.C64:
+            continue
+        if t == 4 and (dtype == tp.DType.C64 or (dtype == tp.DType.C32 and method == tp.ComplexMethod.ONE_M)):
+            with pytest.raises(tp.TprimsError):
+                g.set_tile(t)
             continue
         g.set_tile(t)
         for d in D:
