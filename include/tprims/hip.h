@@ -47,7 +47,9 @@ enum {
 typedef struct {
   int32_t complex_method; /* TPRIMS_CPLX_* */
   int32_t precision;      /* TPRIMS_PREC_* */
-  int32_t tile;           /* 0: chosen by the planner's tuner; else a tile id (see report) */
+  int32_t tile;           /* 0: chosen by the planner's tuner; else a tile id (see report): storage
+                             1 64x64, 2 32x32, 3 128x64, 4 128x128 (LDS-DMA), 5 32x32 on four K groups
+                             (LDS-DMA; runs as 2 where its gathers do not fit) */
   int32_t split_k;        /* 0: chosen by the planner; 1: never split K; n > 1: n K ranges
                              (storage precision; partial tiles reduced in split order) */
   int32_t reserved[4];
