@@ -170,10 +170,21 @@ def config_f64_24(torch, tp, np, ex, stream, steps):
     Bn = B.cpu().numpy().reshape([e] * 4, order="F")
     want = np.einsum("aebf,fced->abcd", An, Bn)
     got = D.cpu().numpy().reshape([e] * 4, order="F")
-    return {"workload": "f64 D[a,b,c,d]=sum_{e,f} A[a,e,b,f] B[f,c,e,d], extents 24", "ms": round(ms, 4),
-            "tflops": round(flops / ms / 1e9, 3), "frac_fp64_peak": round(flops / ms / 1e9 / PEAK_TFLOPS["f64"], 4),
-            "rel_err_vs_fp64": rel_err(np, got, want), "rel_err_elementwise": elem_err(np, got, want),
-            "kernel": plan.report()["kernel"]}
+    out = {"workload": "f64 D[a,b,c,d]=sum_{e,f} A[a,e,b,f] B[f,c,e,d], extents 24", "ms": round(ms, 4),
+           "tflops": round(flops / ms / 1e9, 3), "frac_fp64_peak": round(flops / ms / 1e9 / PEAK_TFLOPS["f64"], 4),
+           "rel_err_vs_fp64": rel_err(np, got, want), "rel_err_elementwise": elem_err(np, got, want),
+           "kernel": plan.report()["kernel"]}
+    # The same contraction on a plan autotuned on these operands
+    # (tprims_product_autotune; the process's tuning cache is cleared after).
+    tuned = tp.Plan(prob)
+    tuned.autotune(1.0, A.data_ptr(), B.data_ptr(), 0.0, None, D.data_ptr(), ex, reps=20)
+    tp.tuning_cache_clear()
+    run_t = lambda: tuned.execute_raw(1.0, A.data_ptr(), B.data_ptr(), 0.0, None, D.data_ptr(), ex)
+    ms_t = timed(torch, run_t, stream, max(steps, 50), 5)
+    got_t = D.cpu().numpy().reshape([e] * 4, order="F")
+    out["autotuned"] = {"ms": round(ms_t, 4), "tflops": round(flops / ms_t / 1e9, 3),
+                        "kernel": tuned.report()["kernel"], "rel_err_vs_fp64": rel_err(np, got_t, want)}
+    return out
 
 
 def config_f64_gemm(torch, tp, np, ex, stream, steps, n=4096):

@@ -109,9 +109,50 @@ typedef struct {
   int32_t d_m_major;        /* write-back runs along M (1) or N (0) */
   int32_t m_axes, n_axes, k_axes, h_axes; /* folded axis counts */
   int32_t strategy;         /* 0 contraction kernel, 1 output pass only, 2 empty, 3 elementwise */
+  int32_t source;           /* tile and K split from: 0 the planner's cost model, 1 the configuration,
+                               2 the tuning cache, 3 tprims_product_autotune on this plan */
   char kernel[64];
 } tprims_plan_report;
 TAPP_EXPORT TAPP_error tprims_product_report(TAPP_tensor_product plan, tprims_plan_report* out);
+
+/* ---- autotuning and the tuning cache ---------------------------------------
+ * The reference resolves a kernel family and blocking once per plan and
+ * freezes it there (crates/tprims-contract/src/select/resolve.rs:164). Here a
+ * plan takes its tile and K split from the planner's cost model, or from the
+ * process's tuning cache when that holds an entry for the plan's reduced
+ * problem (dtype, complex method, precision and every folded axis with its
+ * strides) and the plan asked for neither a tile nor a split.
+ *
+ * tprims_product_autotune times every tile (and, at the storage precision,
+ * K split or none) the plan can run, `reps` executions each after one
+ * warm-up (HIP events on the executor's stream), on the caller's operands
+ * with the arguments of TAPP_execute_product, freezes the fastest into the
+ * plan and records it in the tuning cache. On return D holds the result of
+ * one execution on the chosen configuration: an output that is also read
+ * (C overlapping D with beta != 0) is saved before the timing runs and
+ * restored first. Operands must be device memory of the executor's device
+ * (else TPRIMS_ERR_DEVICE). It changes the plan: no other call may use the
+ * plan meanwhile. best_* (each may be NULL) receive the tile id, the number
+ * of K ranges and the chosen configuration's mean time in ms.
+ *
+ * The cache is kept in the process; tprims_tuning_cache_save writes it to
+ * `path` (replacing the file as a whole), tprims_tuning_cache_load merges a
+ * saved file into it (all entries or, on any malformed line, none:
+ * TPRIMS_ERR_INVALID_ARGUMENT with the line number in the last error). The
+ * file is text: a header line, then one "tile split_k key" line per entry.
+ * Entries apply to plans created after they are stored or loaded. Nothing is
+ * read from the environment. */
+TAPP_EXPORT TAPP_error tprims_product_autotune(TAPP_tensor_product plan, TAPP_executor exec, const void* alpha,
+                                               const void* A, const void* B, const void* beta, const void* C,
+                                               void* D, int32_t reps, int32_t* best_tile, int32_t* best_split,
+                                               double* best_ms);
+/* The plan's tuning-cache key: writes min(length, cap - 1) bytes plus a
+ * terminating zero to out (null allowed when cap is 0) and the length to *len. */
+TAPP_EXPORT TAPP_error tprims_product_tuning_key(TAPP_tensor_product plan, char* out, int64_t cap, int64_t* len);
+TAPP_EXPORT TAPP_error tprims_tuning_cache_save(const char* path);
+TAPP_EXPORT TAPP_error tprims_tuning_cache_load(const char* path);
+TAPP_EXPORT TAPP_error tprims_tuning_cache_clear(void);
+TAPP_EXPORT TAPP_error tprims_tuning_cache_size(int64_t* count);
 
 /* ---- scatter vectors ----------------------------------------------------------
  * The reference exposes these as Rust API: build_scatter and

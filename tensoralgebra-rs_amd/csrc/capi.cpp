@@ -185,6 +185,7 @@ void fill_report(const Plan* p, tprims_plan_report* out) {
   out->k_axes = (int32_t)p->k_ax.size();
   out->h_axes = (int32_t)p->h_ax.size();
   out->strategy = p->strategy;
+  out->source = p->source;
   std::string k = p->kernel_name();
   strncpy(out->kernel, k.c_str(), sizeof out->kernel - 1);
 }

@@ -113,6 +113,8 @@ Axis scaled(const Axis& a, int64_t extent, int64_t by) {
   return {extent, a.sa * by, a.sb * by, a.sc * by, a.sd * by};
 }
 
+}  // namespace
+
 bool tile_allowed(int id, DType t, int prec, int method) {
   if (prec == PREC_FP8) return id == (is_complex(t) ? STILE_128x64 : STILE_256x128);
   if (prec != PREC_STORAGE)
@@ -124,6 +126,8 @@ bool tile_allowed(int id, DType t, int prec, int method) {
   if (id == TILE_128x64 && t == DType::C64) return false;
   return tile_cfg(id) != nullptr;
 }
+
+namespace {
 
 int choose_tile(const Plan& p) {
   const int prec = p.cfg.precision;
@@ -163,9 +167,19 @@ int choose_tile(const Plan& p) {
 
 }  // namespace
 
-void configure(Plan* plan, const PlanConfig& cfg) {
+void configure(Plan* plan, const PlanConfig& cfg_in) {
   Plan& p = *plan;
   const Problem& pr = p.prob;
+  PlanConfig cfg = cfg_in;
+  p.source = cfg.tile != 0 || cfg.split_k != 0 ? 1 : 0;
+  if (p.strategy == 0 && cfg.tile == 0 && cfg.split_k == 0) {
+    int t = 0, s = 0;
+    if (tuning_lookup(tuning_key(p, cfg), &t, &s) && tile_allowed(t, pr.dtype, cfg.precision, cfg.complex_method)) {
+      cfg.tile = t;
+      cfg.split_k = s;
+      p.source = 2;
+    }
+  }
   p.cfg = cfg;
   if (cfg.precision != PREC_STORAGE && is_double(pr.dtype))
     fail(ERR_UNSUPPORTED, "split-BF16 / FP8 precision applies to f32 and c32 only");

@@ -59,6 +59,9 @@ struct Plan {
   // out-of-range sentinel to a multiple of 64 entries (ContractArgs::ak32).
   std::vector<uint32_t> ak32, bk32;
   int tile = 0;
+  // Where the tile and K split came from: 0 the planner's cost model, 1 the
+  // configuration, 2 the tuning cache, 3 autotuning of this plan.
+  int source = 0;
   // 0 contraction (MFMA tiles), 1 output pass only (K empty), 2 output empty,
   // 3 elementwise (all-batch: tiny M x N and K, many batch entries)
   int strategy = 0;
@@ -102,8 +105,20 @@ struct Plan {
 
 // Builds the plan of a validated problem.
 Plan* make_plan(const Problem& p, const PlanConfig& cfg);
-// Re-runs the tile choice and K ordering for a new configuration.
+// Re-runs the tile choice and K ordering for a new configuration. With no
+// tile and no K split asked for, a tuning cache entry for the plan's reduced
+// problem supplies them.
 void configure(Plan* plan, const PlanConfig& cfg);
+// Whether tile `id` exists for this dtype, precision and complex method.
+bool tile_allowed(int id, DType t, int prec, int method);
+
+// The tuning cache (tune.cpp): the tile and K split configuration that
+// autotuning chose for a reduced problem, keyed by tuning_key (dtype,
+// complex method, precision, strategy and every folded axis with its
+// strides; the tile and split of `cfg` are not part of it).
+std::string tuning_key(const Plan& p, const PlanConfig& cfg);
+bool tuning_lookup(const std::string& key, int* tile, int* split_k);
+void tuning_store(const std::string& key, int tile, int split_k);
 
 // The scatter vector of a mixed-radix index group, fastest axis first
 // (crates/tprims-kernel/src/pack/scatter.rs:70).

@@ -13,6 +13,7 @@ nothing here computes a contraction in Python.
 """
 import array
 import ctypes
+import os
 import enum
 
 from . import _capi
@@ -21,7 +22,8 @@ from ._capi import TprimsError, check, i64_array, lib
 __all__ = [
     "DType", "Op", "LayoutSpec", "OperandSpec", "CSpec", "Labels", "DotGeneral", "Problem", "PlanConfig", "Plan",
     "Executor", "TprimsError", "ComplexMethod", "Precision", "col_major", "row_major", "build_scatter",
-    "build_block_scatter", "IRREGULAR", "SCATTERS",
+    "build_block_scatter", "IRREGULAR", "SCATTERS", "tuning_cache_save", "tuning_cache_load",
+    "tuning_cache_clear", "tuning_cache_size",
 ]
 
 
@@ -343,6 +345,26 @@ def build_block_scatter(scat, blk):
     return _sized(lambda out, cap, n: lib().tprims_build_block_scatter(v, len(scat), int(blk), out, cap, n))
 
 
+def tuning_cache_save(path):
+    """Writes the process's tuning cache to `path` (tprims_tuning_cache_save)."""
+    check(lib().tprims_tuning_cache_save(os.fsencode(path)))
+
+
+def tuning_cache_load(path):
+    """Merges a saved tuning cache into the process's (all entries or none)."""
+    check(lib().tprims_tuning_cache_load(os.fsencode(path)))
+
+
+def tuning_cache_clear():
+    check(lib().tprims_tuning_cache_clear())
+
+
+def tuning_cache_size():
+    n = ctypes.c_int64(0)
+    check(lib().tprims_tuning_cache_size(ctypes.byref(n)))
+    return n.value
+
+
 def _scalar(v, dtype):
     v = complex(v)
     if dtype == DType.F32:
@@ -405,6 +427,29 @@ class Plan:
         out = {f: getattr(r, f) for f, _ in _capi.PlanReportC._fields_}
         out["kernel"] = r.kernel.decode()
         return out
+
+    def autotune(self, alpha, a, b, beta, c, d, executor=None, reps=3):
+        """tprims_product_autotune: times every tile (and K split or none) this
+        plan can run on the given device operands (element addresses of index
+        zero, as execute_raw takes them), freezes the fastest into the plan,
+        records it in the process's tuning cache and leaves D as one execution
+        does. Returns (tile id, K ranges, mean ms of the chosen configuration)."""
+        dt = self.problem.dtype
+        pa, _ka = _scalar(alpha, dt)
+        pb, _kb = _scalar(beta, dt)
+        t, s, ms = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_double(0)
+        ex = executor.handle if executor is not None else 0
+        check(lib().tprims_product_autotune(self.handle, ex, pa, a, b, pb, c, d, int(reps), ctypes.byref(t),
+                                            ctypes.byref(s), ctypes.byref(ms)))
+        return t.value, s.value, ms.value
+
+    def tuning_key(self):
+        """The key of this plan's reduced problem in the tuning cache."""
+        n = ctypes.c_int64(0)
+        check(lib().tprims_product_tuning_key(self.handle, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        check(lib().tprims_product_tuning_key(self.handle, buf, n.value + 1, ctypes.byref(n)))
+        return buf.value.decode()
 
     def execute_raw(self, alpha, a, b, beta, c, d, executor=None):
         """Execute on element addresses of index zero (plan/mod.rs:452)."""

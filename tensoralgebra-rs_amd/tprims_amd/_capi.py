@@ -55,7 +55,7 @@ class PlanReportC(ctypes.Structure):
                 ("tile_n", ctypes.c_int32), ("tile_k", ctypes.c_int32), ("a_k_major", ctypes.c_int32),
                 ("b_k_major", ctypes.c_int32), ("d_m_major", ctypes.c_int32), ("m_axes", ctypes.c_int32),
                 ("n_axes", ctypes.c_int32), ("k_axes", ctypes.c_int32), ("h_axes", ctypes.c_int32),
-                ("strategy", ctypes.c_int32), ("kernel", ctypes.c_char * 64)]
+                ("strategy", ctypes.c_int32), ("source", ctypes.c_int32), ("kernel", ctypes.c_char * 64)]
 
 
 class DLDevice(ctypes.Structure):
@@ -97,6 +97,8 @@ EXPORTED = [
     "tprims_product_report", "tprims_execute_product_noalloc", "tprims_execute_product_dlpack", "tprims_group_create", "tprims_group_execute",
     "tprims_group_use_graph", "tprims_group_execute_each", "tprims_group_info", "tprims_group_set_tile",
     "tprims_group_autotune", "tprims_group_destroy", "tprims_product_vjp", "tprims_product_vjp_report",
+    "tprims_product_autotune", "tprims_product_tuning_key", "tprims_tuning_cache_save", "tprims_tuning_cache_load",
+    "tprims_tuning_cache_clear", "tprims_tuning_cache_size",
 ]
 
 
@@ -167,6 +169,14 @@ def _declare(lib):
         "tprims_group_set_tile": (c_int, [intptr, ctypes.c_int32]),
         "tprims_group_autotune": (c_int, [intptr, intptr, c_int, ctypes.POINTER(ctypes.c_int32),
                                           ctypes.POINTER(ctypes.c_double)]),
+        "tprims_product_autotune": (c_int, [intptr, intptr, vp, vp, vp, vp, vp, vp, ctypes.c_int32,
+                                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                            ctypes.POINTER(ctypes.c_double)]),
+        "tprims_product_tuning_key": (c_int, [intptr, ctypes.c_char_p, ctypes.c_int64, i64p]),
+        "tprims_tuning_cache_save": (c_int, [ctypes.c_char_p]),
+        "tprims_tuning_cache_load": (c_int, [ctypes.c_char_p]),
+        "tprims_tuning_cache_clear": (c_int, []),
+        "tprims_tuning_cache_size": (c_int, [i64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
