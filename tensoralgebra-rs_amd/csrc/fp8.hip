@@ -22,7 +22,11 @@
 //  1. k_pack_fp8: one wave per (batch h, 128-k block, 16-row block): each lane
 //     gathers its 32 values, the four lanes of a row reduce the panel's amax
 //     with cross-lane shuffles, and the lane stores its 32 bytes of the 2 KiB
-//     fragment plus (lanes of the first quarter) the row's int32 scale word.
+//     fragment as two 16-byte halves, at 16 lane and 1024 + 16 lane, so that
+//     each 16-byte LDS read of the GEMM hits 64 distinct banks per 16 lanes
+//     (at 32 lane, lanes 0 and 8 of every 16 shared banks: 1.05e7 bank-conflict
+//     cycles per 4096^3 GEMM), plus (lanes of the first quarter) the row's
+//     int32 scale word.
 //  2. k_gemm_fp8: per 128-k step the workgroup copies its row blocks'
 //     fragments and scale words into LDS by DMA; each wave reads 32 bytes per
 //     fragment and runs one scaled MFMA per 16x16 block and real product.
@@ -123,7 +127,7 @@ __global__ __launch_bounds__(256) void k_pack_fp8(F8PackArgs p) {
       }
     }
   }
-  uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rb) * COMPS * F8_FRAG + lane * 32;
+  uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rb) * COMPS * F8_FRAG + lane * 16;
 #pragma unroll
   for (int c = 0; c < COMPS; c++) {
     const bool neg = c == 1 && p.conj;
@@ -141,9 +145,8 @@ __global__ __launch_bounds__(256) void k_pack_fp8(F8PackArgs p) {
       int x = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, 0, false);
       w[d] = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, x, true);
     }
-    v4u* dst = (v4u*)(out + c * F8_FRAG);
-    dst[0] = v4u{(unsigned)w[0], (unsigned)w[1], (unsigned)w[2], (unsigned)w[3]};
-    dst[1] = v4u{(unsigned)w[4], (unsigned)w[5], (unsigned)w[6], (unsigned)w[7]};
+    *(v4u*)(out + c * F8_FRAG) = v4u{(unsigned)w[0], (unsigned)w[1], (unsigned)w[2], (unsigned)w[3]};
+    *(v4u*)(out + c * F8_FRAG + 1024) = v4u{(unsigned)w[4], (unsigned)w[5], (unsigned)w[6], (unsigned)w[7]};
     if (lane < 16) p.scales[((h * p.KB + kb) * p.RB * 16 + r) * COMPS + c] = 127 + e;
   }
 }
@@ -200,7 +203,7 @@ __global__ __launch_bounds__(256) void k_pack_fp8_lds(F8PackArgs p) {
       }
     }
   }
-  uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rb) * COMPS * F8_FRAG + lane * 32;
+  uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rb) * COMPS * F8_FRAG + lane * 16;
 #pragma unroll
   for (int c = 0; c < COMPS; c++) {
     const bool neg = c == 1 && p.conj;
@@ -218,9 +221,8 @@ __global__ __launch_bounds__(256) void k_pack_fp8_lds(F8PackArgs p) {
       int x = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, 0, false);
       wd[d] = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, x, true);
     }
-    v4u* dst = (v4u*)(out + c * F8_FRAG);
-    dst[0] = v4u{(unsigned)wd[0], (unsigned)wd[1], (unsigned)wd[2], (unsigned)wd[3]};
-    dst[1] = v4u{(unsigned)wd[4], (unsigned)wd[5], (unsigned)wd[6], (unsigned)wd[7]};
+    *(v4u*)(out + c * F8_FRAG) = v4u{(unsigned)wd[0], (unsigned)wd[1], (unsigned)wd[2], (unsigned)wd[3]};
+    *(v4u*)(out + c * F8_FRAG + 1024) = v4u{(unsigned)wd[4], (unsigned)wd[5], (unsigned)wd[6], (unsigned)wd[7]};
     if (lane < 16) p.scales[((h * p.KB + kb) * p.RB * 16 + r) * COMPS + c] = 127 + e;
   }
 }
@@ -325,15 +327,15 @@ __global__ __launch_bounds__(G::NT) void k_gemm_fp8(F8GemmArgs g) {
     for (int c = 0; c < COMPS; c++) {
 #pragma unroll
       for (int i = 0; i < G::MI; i++) {
-        const v4u* q = (const v4u*)(stage + (size_t)((((wm >> 4) + i) * COMPS + c) * F8_FRAG) + lane * 32);
-        const v4u lo = q[0], hi = q[1];
+        const char* q = stage + (size_t)((((wm >> 4) + i) * COMPS + c) * F8_FRAG) + lane * 16;
+        const v4u lo = *(const v4u*)q, hi = *(const v4u*)(q + 1024);
         a[c][i] = v8i{(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3], (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
         sa[c][i] = sc[(wm + 16 * i + lr) * COMPS + c];
       }
 #pragma unroll
       for (int j = 0; j < G::NI; j++) {
-        const v4u* q = (const v4u*)(stage + (size_t)((G::A_FRAGS + ((wn >> 4) + j) * COMPS + c) * F8_FRAG) + lane * 32);
-        const v4u lo = q[0], hi = q[1];
+        const char* q = stage + (size_t)((G::A_FRAGS + ((wn >> 4) + j) * COMPS + c) * F8_FRAG) + lane * 16;
+        const v4u lo = *(const v4u*)q, hi = *(const v4u*)(q + 1024);
         b[c][j] = v8i{(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3], (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
         sb[c][j] = sc[BM * COMPS + (wn + 16 * j + lr) * COMPS + c];
       }

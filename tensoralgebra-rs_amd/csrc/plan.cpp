@@ -392,6 +392,7 @@ void configure(Plan* plan, const PlanConfig& cfg_in) {
   // 256x128 (one workgroup per CU): f32 transform step, 3 terms, 135 -> 157
   // TFLOP/s. Both tiles have 32-k steps, so the K order above stands.
   if (cfg.tile == 0 && p.tile == STILE_256x128 && p.off32 &&
+      (cfg.precision == PREC_SPLIT_BF16_2 || cfg.precision == PREC_SPLIT_BF16_3) &&
       split_a_in_gemm(false, p.a_kmajor, p.a_vec16, p.N, STILE_128x128))
     p.tile = STILE_128x128;
 }
