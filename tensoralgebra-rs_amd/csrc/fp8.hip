@@ -246,7 +246,10 @@ struct F8Cfg {
   static constexpr int MI = WM / 16, NI = WN / 16;
   static constexpr int LDD = BN + 1;
   static constexpr size_t LDS_MAIN = 2 * STAGE_BYTES;
-  static constexpr size_t LDS_EPI = (size_t)COMPS * BM * LDD * 4;
+  // The staged epilogue holds the whole tile when it fits the main loop's
+  // LDS, else one wave row (WM rows) at a time.
+  static constexpr int EPI_ROWS = (size_t)COMPS * BM * LDD * 4 <= LDS_MAIN ? BM : WM;
+  static constexpr size_t LDS_EPI = (size_t)COMPS * EPI_ROWS * LDD * 4;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
   static_assert(PIECES % NW == 0, "whole fragment pieces per wave");
   static_assert(A_SC % 256 == 0 && B_SC % 256 == 0, "scale pieces");
@@ -359,32 +362,70 @@ __global__ __launch_bounds__(G::NT) void k_gemm_fp8(F8GemmArgs g) {
     if (kb + 2 < nk) issue(kb + 2, (int)(kb & 1));
   }
 
-  float* Dre = (float*)smem;
-  float* Dim = Dre + BM * G::LDD;
   const int lk = lane >> 4;
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  if constexpr (!G::CPLX) {
+    // D contiguous along M with affine offsets, no broadcast copies and
+    // 16-byte aligned columns: a lane's four consecutive accumulator rows go
+    // straight to D as one 16-byte store (alpha and beta * C in registers).
+    float* __restrict__ Dg = (float*)p.D + p.hd[h];
+    const float* __restrict__ Cg = (const float*)p.C + p.hc[h];
+    const bool direct = p.d_mmajor && p.mn_affine && p.dr == nullptr && p.cd_stride[2] == 1 &&
+                        p.cd_stride[3] % 4 == 0 && (uintptr_t)Dg % 16 == 0 &&
+                        (!p.read_c || (p.cd_stride[0] == 1 && p.cd_stride[1] % 4 == 0 && (uintptr_t)Cg % 16 == 0));
+    if (direct) {
+      const float alr = (float)p.alpha[0], ber = (float)p.beta[0];
 #pragma unroll
-  for (int i = 0; i < G::MI; i++)
+      for (int i = 0; i < G::MI; i++)
 #pragma unroll
-    for (int j = 0; j < G::NI; j++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int row = wm + 16 * i + 4 * lk + q, col = wn + 16 * j + lr;
-        if constexpr (G::CPLX) {
-          Dre[row * G::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
-          Dim[row * G::LDD + col] = acc[2][i][j][q];
-        } else {
-          Dre[row * G::LDD + col] = acc[0][i][j][q];
+        for (int j = 0; j < G::NI; j++) {
+          const int64_t m = m0 + wm + 16 * i + 4 * lk, n = n0 + wn + 16 * j + lr;
+          if (m >= p.M || n >= p.N) continue;
+          const int64_t off = m + n * p.cd_stride[3], coff = m + n * p.cd_stride[1];
+          if (p.M - m >= 4) {
+            v4f y = alr * acc[0][i][j];
+            if (p.read_c) y += ber * *(const v4f*)(Cg + coff);
+            *(v4f*)(Dg + off) = y;
+          } else {
+            for (int q = 0; q < p.M - m; q++)
+              Dg[off + q] = alr * acc[0][i][j][q] + (p.read_c ? ber * Cg[coff + q] : 0.0f);
+          }
         }
-      }
-  __syncthreads();
-  dev::write_back<float, float, G::CPLX, BM, BN, G::NT, G::LDD>(p, h, (int64_t)tm * BM, (int64_t)tn * BN, Dre, Dim);
+      return;
+    }
+  }
+  constexpr int ER = G::EPI_ROWS;
+  float* Dre = (float*)smem;
+  float* Dim = Dre + ER * G::LDD;
+#pragma unroll 1
+  for (int c = 0; c < BM / ER; c++) {
+    if (c > 0) __syncthreads();  // the previous chunk's write-back has read the staging rows
+    if (wm / ER == c) {
+#pragma unroll
+      for (int i = 0; i < G::MI; i++)
+#pragma unroll
+        for (int j = 0; j < G::NI; j++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int row = wm - c * ER + 16 * i + 4 * lk + q, col = wn + 16 * j + lr;
+            if constexpr (G::CPLX) {
+              Dre[row * G::LDD + col] = acc[0][i][j][q] - acc[1][i][j][q];
+              Dim[row * G::LDD + col] = acc[2][i][j][q];
+            } else {
+              Dre[row * G::LDD + col] = acc[0][i][j][q];
+            }
+          }
+    }
+    __syncthreads();
+    dev::write_back<float, float, G::CPLX, ER, BN, G::NT, G::LDD>(p, h, m0 + c * ER, n0, Dre, Dim);
+  }
 }
 
 struct Shape {
   int bm, bn;
 };
 
-Shape f8_shape(bool cplx) { return cplx ? Shape{128, 64} : Shape{256, 128}; }
+Shape f8_shape(bool cplx) { return cplx ? Shape{128, 64} : Shape{256, 256}; }
 
 int64_t pad_rb(int64_t rows, int bm) {
   const int64_t rb = (rows + bm - 1) / bm * (bm / 16);
@@ -421,7 +462,9 @@ hipError_t run(const ContractArgs& a, void* panels, hipStream_t s) {
     typedef F8Cfg<true, 128, 64, 32, 32> G;
     hipLaunchKernelGGL(k_gemm_fp8<G>, dim3((unsigned)nwg), dim3(G::NT), 0, s, g);
   } else {
-    typedef F8Cfg<false, 256, 128, 64, 64> G;
+    // 256 x 256 on four waves of 128 x 128 (one per SIMD): half the LDS
+    // fragment reads per MFMA of 64 x 64 wave tiles.
+    typedef F8Cfg<false, 256, 256, 128, 64> G;
     hipLaunchKernelGGL(k_gemm_fp8<G>, dim3((unsigned)nwg), dim3(G::NT), 0, s, g);
   }
   return hipGetLastError();

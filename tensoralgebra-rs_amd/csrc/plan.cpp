@@ -116,7 +116,7 @@ Axis scaled(const Axis& a, int64_t extent, int64_t by) {
 }  // namespace
 
 bool tile_allowed(int id, DType t, int prec, int method) {
-  if (prec == PREC_FP8) return id == (is_complex(t) ? STILE_128x64 : STILE_256x128);
+  if (prec == PREC_FP8) return id == (is_complex(t) ? STILE_128x64 : STILE_256x256);
   if (prec != PREC_STORAGE)
     return id == STILE_64x64 || (id == STILE_128x64 && is_complex(t)) ||
            ((id == STILE_128x128 || id == STILE_256x128) && !is_complex(t));
@@ -137,7 +137,7 @@ int choose_tile(const Plan& p) {
   }
   std::vector<std::pair<int, double>> cands;
   if (prec == PREC_FP8) {
-    return is_complex(p.prob.dtype) ? STILE_128x64 : STILE_256x128;
+    return is_complex(p.prob.dtype) ? STILE_128x64 : STILE_256x256;
   } else if (prec != PREC_STORAGE) {
     cands = {{STILE_256x128, 1.3}, {STILE_128x128, 1.15}, {STILE_128x64, 1.2}, {STILE_64x64, 1.0}};
   } else {

@@ -99,7 +99,7 @@ void replan(Plan& p, const PlanConfig& cfg) {
 // and complex method).
 std::vector<int> tile_candidates(int prec) {
   if (prec == PREC_STORAGE) return {TILE_128x128, TILE_128x64, TILE_64x64, TILE_32x32, TILE_32x32K};
-  if (prec == PREC_FP8) return {STILE_256x128, STILE_128x64};
+  if (prec == PREC_FP8) return {STILE_256x256, STILE_128x64};
   return {STILE_256x128, STILE_128x128, STILE_128x64, STILE_64x64};
 }
 
