@@ -603,7 +603,7 @@ const TileCfg* tile_cfg(int id) {
       {STILE_64x64, 64, 64, 32, 256, "STILE_64x64"},
       {STILE_256x128, 256, 128, 32, 512, "STILE_256x128"},
       {STILE_128x64, 128, 64, 32, 512, "STILE_128x64"},
-      {STILE_256x256, 256, 256, 128, 256, "STILE_256x256"},
+      {STILE_256x256, 256, 256, 128, 512, "STILE_256x256"},
   };
   for (auto& t : tiles)
     if (t.id == id) return &t;

@@ -59,6 +59,7 @@ struct F8PackArgs {
   int32_t conj;
   int32_t vec;     // ContractArgs::a_kvec / b_kvec of this operand
   int32_t kmajor;  // ContractArgs::a_kmajor / b_kmajor: contiguous along K
+  int32_t rvec;    // rows in 16-byte runs of 4 consecutive f32 (M- or N-major, 16-byte aligned)
   uint8_t* out;
   int32_t* scales;  // [h][kb][RB * 16 rows][comp]
 };
@@ -224,6 +225,125 @@ __global__ __launch_bounds__(256) void k_pack_fp8_lds(F8PackArgs p) {
     *(v4u*)(out + c * F8_FRAG) = v4u{(unsigned)wd[0], (unsigned)wd[1], (unsigned)wd[2], (unsigned)wd[3]};
     *(v4u*)(out + c * F8_FRAG + 1024) = v4u{(unsigned)wd[4], (unsigned)wd[5], (unsigned)wd[6], (unsigned)wd[7]};
     if (lane < 16) p.scales[((h * p.KB + kb) * p.RB * 16 + r) * COMPS + c] = 127 + e;
+  }
+}
+
+// The same panels for operands gathered along their rows (M- or N-major):
+// the workgroup's four waves take the same 64 rows, one lane per row, and
+// one 32-k quarter of the 128-k block each, so every load instruction reads
+// 64 rows at one k (consecutive addresses where the row scatter is); the
+// waves combine the rows' panel maxima through LDS, and each lane stores its
+// quarter's 32 bytes into the fragment slot of (row, quarter).
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_pack_fp8_rows(F8PackArgs p) {
+  typedef Store<float, CPLX> St;
+  typedef typename St::S S;
+  constexpr int COMPS = CPLX ? 2 : 1;
+  __shared__ float red[COMPS][4][64];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;  // q: this wave's 32-k quarter
+  const int64_t rgroups = p.RB / 4;
+  int64_t bid = blockIdx.x;
+  const int64_t rg = bid % rgroups;
+  bid /= rgroups;
+  const int64_t kb = bid % p.KB;
+  const int64_t h = bid / p.KB;
+  const int64_t r = rg * 64 + lane;
+  const bool rok = r < p.R;
+  const int64_t k0 = kb * F8_KBLK + 32 * q;
+  // Row 0 and k 0 have offset 0: out-of-range lanes read X[0] and store zero.
+  const S* __restrict__ X = (const S*)p.X + p.hv[h] + (rok ? p.rv[r] : 0);
+  float re[32], im[32];
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    const int64_t k = k0 + j;
+    const bool ok = rok && k < p.K;
+    const S v = X[k < p.K ? p.kv[k] : 0];
+    re[j] = ok ? St::re(v) : 0.0f;
+    im[j] = ok ? St::im(v) : 0.0f;
+  }
+#pragma unroll
+  for (int c = 0; c < COMPS; c++) {
+    float amax = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 32; j++) amax = fmaxf(amax, fabsf(c == 0 ? re[j] : im[j]));
+    red[c][q][lane] = amax;
+  }
+  __syncthreads();
+  // Fragment of row block r / 16; this lane's slot is (quarter q, row r % 16).
+  uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rg * 4 + lane / 16) * COMPS * F8_FRAG + (q * 16 + lane % 16) * 16;
+#pragma unroll
+  for (int c = 0; c < COMPS; c++) {
+    const float amax = fmaxf(fmaxf(red[c][0][lane], red[c][1][lane]), fmaxf(red[c][2][lane], red[c][3][lane]));
+    const int e = panel_exponent(amax);
+    const bool neg = c == 1 && p.conj;
+    const float inv = ldexpf(neg ? -1.0f : 1.0f, -e);
+    int w[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+      const float* v = (c == 0 ? re : im) + 4 * d;
+      int x = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * inv, v[1] * inv, 0, false);
+      w[d] = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * inv, v[3] * inv, x, true);
+    }
+    *(v4u*)(out + c * F8_FRAG) = v4u{(unsigned)w[0], (unsigned)w[1], (unsigned)w[2], (unsigned)w[3]};
+    *(v4u*)(out + c * F8_FRAG + 1024) = v4u{(unsigned)w[4], (unsigned)w[5], (unsigned)w[6], (unsigned)w[7]};
+    if (q == 0) p.scales[((h * p.KB + kb) * p.RB * 16 + r) * COMPS + c] = 127 + e;
+  }
+}
+
+// Real operands whose rows come in 16-byte runs (4 consecutive f32 at
+// 16-byte aligned offsets): each lane loads 4 rows at a time, so a load
+// instruction reads 1 KiB (256 rows at one k); the workgroup's four waves
+// take the same 256 rows and one 32-k quarter each and combine the rows'
+// panel maxima through LDS, and each lane stores its four rows' 32-byte
+// quarters into their fragment slots. Rows past the padded panel (RB * 16)
+// are not stored.
+__global__ __launch_bounds__(256) void k_pack_fp8_rows4(F8PackArgs p) {
+  __shared__ float red[4][256];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t rgroups = (p.RB + 15) / 16;
+  int64_t bid = blockIdx.x;
+  const int64_t rg = bid % rgroups;
+  bid /= rgroups;
+  const int64_t kb = bid % p.KB;
+  const int64_t h = bid / p.KB;
+  const int64_t r0 = rg * 256 + 4 * lane;  // the lane's first row
+  const bool rok = r0 < p.R;               // R is a multiple of 4: all four rows or none
+  const int64_t k0 = kb * F8_KBLK + 32 * q;
+  const float* __restrict__ X = (const float*)p.X + p.hv[h] + (rok ? p.rv[r0] : 0);
+  v4f v[32];
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    const int64_t k = k0 + j;
+    const bool ok = rok && k < p.K;
+    const v4f x = *(const v4f*)(X + (k < p.K ? p.kv[k] : 0));
+    v[j] = ok ? x : v4f{0, 0, 0, 0};
+  }
+  v4f amax = v4f{0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 32; j++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) amax[i] = fmaxf(amax[i], fabsf(v[j][i]));
+#pragma unroll
+  for (int i = 0; i < 4; i++) red[q][4 * lane + i] = amax[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int rl = 4 * lane + i;  // row within the group of 256
+    const int64_t r = rg * 256 + rl;
+    if (r >= p.RB * 16) continue;
+    const float m = fmaxf(fmaxf(red[0][rl], red[1][rl]), fmaxf(red[2][rl], red[3][rl]));
+    const int e = panel_exponent(m);
+    const float inv = ldexpf(1.0f, -e);
+    int w[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+      int x = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * d][i] * inv, v[4 * d + 1][i] * inv, 0, false);
+      w[d] = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * d + 2][i] * inv, v[4 * d + 3][i] * inv, x, true);
+    }
+    uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + r / 16) * F8_FRAG + (q * 16 + rl % 16) * 16;
+    *(v4u*)out = v4u{(unsigned)w[0], (unsigned)w[1], (unsigned)w[2], (unsigned)w[3]};
+    *(v4u*)(out + 1024) = v4u{(unsigned)w[4], (unsigned)w[5], (unsigned)w[6], (unsigned)w[7]};
+    if (q == 0) p.scales[(h * p.KB + kb) * p.RB * 16 + r] = 127 + e;
   }
 }
 
@@ -442,14 +562,21 @@ hipError_t run(const ContractArgs& a, void* panels, hipStream_t s) {
   uint8_t* Bp = Ap + a.H * KB * RBa * COMPS * F8_FRAG;
   int32_t* As = (int32_t*)(Bp + a.H * KB * RBb * COMPS * F8_FRAG);
   int32_t* Bs = As + a.H * KB * RBa * 16 * COMPS;
-  F8PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kvec, a.a_kmajor, Ap, As};
-  F8PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, a.b_kmajor, Bp, Bs};
+  // Rows in 16-byte runs: A alone (a_vec16), B with A (ab_vec); real only.
+  const int a_rvec = !CPLX && !a.a_kmajor && a.a_vec16;
+  const int b_rvec = !CPLX && !a.b_kmajor && a.ab_vec;
+  F8PackArgs pa{a.A, a.am, a.ak, a.ha, a.M, a.K, a.H, RBa, KB, a.conj_a, a.a_kvec, a.a_kmajor, a_rvec, Ap, As};
+  F8PackArgs pb{a.B, a.bn, a.bk, a.hb, a.N, a.K, a.H, RBb, KB, a.conj_b, a.b_kvec, a.b_kmajor, b_rvec, Bp, Bs};
   for (const F8PackArgs* q : {&pa, &pb}) {
     const int64_t grid = q->H * q->KB * (q->RB / 4);
     if (grid > max_grid(256)) return hipErrorInvalidConfiguration;
     if (q->vec) hipLaunchKernelGGL((k_pack_fp8<CPLX, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
+    else if (q->rvec) {
+      const int64_t g4 = q->H * q->KB * ((q->RB + 15) / 16);
+      hipLaunchKernelGGL(k_pack_fp8_rows4, dim3((unsigned)g4), dim3(256), 0, s, *q);
+    }
     else if (q->kmajor) hipLaunchKernelGGL((k_pack_fp8_lds<CPLX>), dim3((unsigned)grid), dim3(256), 0, s, *q);
-    else hipLaunchKernelGGL((k_pack_fp8<CPLX, false>), dim3((unsigned)grid), dim3(256), 0, s, *q);
+    else hipLaunchKernelGGL((k_pack_fp8_rows<CPLX>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -29,7 +29,7 @@ enum SplitTileId : int {
   STILE_64x64 = 12,    // 4 waves of 32x32
   STILE_256x128 = 13,  // 8 waves of 64x64 (f32)
   STILE_128x64 = 14,   // 8 waves of 32x32 (c32)
-  STILE_256x256 = 15,  // FP8 f32: 4 waves of 128x128
+  STILE_256x256 = 15,  // FP8 f32: 8 waves of 128x64
 };
 
 const TileCfg* tile_cfg(int id);
