@@ -31,33 +31,41 @@ def grid_values(dtype, n, seed):
     return v.astype(NP[dtype])
 
 
-def batched_gemm(M, K, N, H, a_kmajor, conj_a=False, conj_b=False):
+def batched_gemm(M, K, N, H, a_kmajor, conj_a=False, conj_b=False, b_kmajor=True):
     a_str = [K, 1, M * K] if a_kmajor else [1, M, M * K]
+    b_str = [1, K, K * N] if b_kmajor else [N, 1, K * N]
     a = T([M, K, H], a_str, [0, 2, 3], conj=conj_a)
-    b = T([K, N, H], [1, K, K * N], [2, 1, 3], conj=conj_b)
+    b = T([K, N, H], b_str, [2, 1, 3], conj=conj_b)
     d = T([M, N, H], [1, M, M * N], [0, 1, 3])
     return a, b, d
 
 
-# (M, K, N, H, A K-major, conj A, conj B, beta)
+# (M, K, N, H, A K-major, B K-major, conj A, conj B, beta). Operands gathered
+# along their rows (not K-major) take the row packers: four rows per lane
+# when the rows come in 16-byte runs (M or N a multiple of 4 here), one row
+# per lane otherwise.
 SHAPES = [
-    (40, 300, 24, 3, True, False, False, 0.0),
-    (40, 300, 24, 3, False, True, False, -0.5),
-    (300, 130, 200, 1, True, False, True, 0.0),
-    (17, 1000, 9, 2, True, True, True, -0.5),
-    (257, 128, 129, 1, False, False, False, 0.0),
-    (5, 7, 3, 4, True, False, False, -0.5),
+    (40, 300, 24, 3, True, True, False, False, 0.0),
+    (40, 300, 24, 3, False, True, True, False, -0.5),
+    (300, 130, 200, 1, True, True, False, True, 0.0),
+    (17, 1000, 9, 2, True, True, True, True, -0.5),
+    (257, 128, 129, 1, False, True, False, False, 0.0),
+    (5, 7, 3, 4, True, True, False, False, -0.5),
+    (40, 300, 24, 3, False, False, False, True, 0.0),
+    (52, 260, 129, 2, False, False, True, False, -0.5),
+    (257, 384, 68, 1, True, False, False, False, 0.0),
 ]
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.C32], ids=lambda d: d.name)
-@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(str(v) for v in s[:4]) + ("K" if s[4] else "M"))
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(str(v) for v in s[:4]) + ("K" if s[4] else "M")
+                         + ("K" if s[5] else "N"))
 def test_fp8_exact_on_representable_inputs(gpu, dtype, shape):
-    M, K, N, H, akm, ca, cb, beta = shape
+    M, K, N, H, akm, bkm, ca, cb, beta = shape
     if not dtype.is_complex and (ca or cb):
         ca = cb = False
-    a, b, d = batched_gemm(M, K, N, H, akm, ca, cb)
+    a, b, d = batched_gemm(M, K, N, H, akm, ca, cb, b_kmajor=bkm)
     av = grid_values(dtype, a.storage(), 1)
     bv = grid_values(dtype, b.storage(), 2)
     dv = np.zeros(d.storage(), NP[dtype])
