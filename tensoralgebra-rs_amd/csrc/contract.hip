@@ -599,6 +599,7 @@ const TileCfg* tile_cfg(int id) {
 #undef TP_ROW
       {TILE_128x128, 128, 128, 16, 512, "TILE_128x128"},  // c64: 1024 threads
       {TILE_32x32K, 32, 32, 16, 256, "TILE_32x32K"},
+      {TILE_32x32R, 32, 32, 16, 512, "TILE_32x32R"},
       {STILE_128x128, 128, 128, 32, 256, "STILE_128x128"},
       {STILE_64x64, 64, 64, 32, 256, "STILE_64x64"},
       {STILE_256x128, 256, 128, 32, 512, "STILE_256x128"},
@@ -619,9 +620,16 @@ hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_
         ((a.M + c->bm - 1) / c->bm) * ((a.N + c->bn - 1) / c->bn) * a.H * (a.ksplit > 1 ? a.ksplit : 1);
     if (nwg <= max_grid(c->threads)) return launch_contract_dma(a, t, method, tile_id, s);
   }
-  // The K-group tile runs where its gathers fit, else as the register-staged
+  // The K-group tiles run where their gathers fit (the register-direct ones
+  // as the LDS-DMA K-group tile otherwise), else as the register-staged
   // 32x32 tile; the DMA-only 128x128 tile falls back to the register-staged
   // 64x64 kernel.
+  if (tile_id == TILE_32x32R) {
+    if (a.ksplit <= 1 && direct_supported(a.off32, a.ab_vec, t) &&
+        ((a.M + 31) / 32) * ((a.N + 31) / 32) * a.H <= max_grid(512))
+      return launch_contract_direct(a, t, s);
+    tile_id = TILE_32x32K;
+  }
   if (tile_id == TILE_32x32K) {
     if (a.ksplit <= 1 && kgroup_supported(a.off32, a.ab_vec, a.a_kmajor, a.b_kmajor, t) &&
         ((a.M + 31) / 32) * ((a.N + 31) / 32) * a.H <= max_grid(256))

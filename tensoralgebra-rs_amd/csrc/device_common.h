@@ -220,7 +220,11 @@ __device__ inline void write_back(const ContractArgs& p, int64_t h, int64_t m0, 
             }
             *(vec*)(D + off) = y;
           } else {
-            for (int q = 0; q < lim; q++) {
+            // A constant trip count keeps x in registers (a variable index
+            // would put it in scratch memory).
+#pragma unroll
+            for (int q = 0; q < V; q++) {
+              if (q >= lim) break;
               T y = alr * x[q];
               if (rc) y += ber * Cp[coff + q];
               D[off + q] = y;

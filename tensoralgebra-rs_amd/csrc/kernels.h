@@ -22,6 +22,7 @@ enum TileId : int {
   TILE_128x64 = 3, // 4 waves of 64x32
   TILE_128x128 = 4, // LDS-DMA kernel only (16 waves of 32x32, or 8 of 64x32: by dtype, schedule, majors)
   TILE_32x32K = 5,  // LDS-DMA kernel only: 4 waves of 32x32, each over every fourth K slab
+  TILE_32x32R = 6,  // f64 in 16-byte runs: 8 waves of 32x32, each over every eighth K slab, loading into registers
 };
 // Tile ids of the split-BF16 kernels (16x16x32 bf16 MFMA family).
 enum SplitTileId : int {
@@ -51,6 +52,11 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int t
 // runs of f64 or c32 elements (any majors) or f32 elements (K-major).
 bool kgroup_supported(bool off32, bool ab_vec, bool a_kmajor, bool b_kmajor, DType t);
 hipError_t launch_contract_kgroup(const ContractArgs& a, DType t, int method, hipStream_t s);
+// The 32x32 tile on eight K groups with operands loaded straight into
+// registers (TILE_32x32R): f64, 32-bit gather offsets, A and B in 16-byte
+// runs (any majors), one K range.
+bool direct_supported(bool off32, bool ab_vec, DType t);
+hipError_t launch_contract_direct(const ContractArgs& a, DType t, hipStream_t s);
 // Split-BF16 for f32/c32 storage (nterms 2 or 3): A and B are split once into
 // bf16 fragment panels (device scratch of split_panel_bytes), then multiplied.
 size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id);
@@ -62,7 +68,7 @@ hipError_t launch_contract_split(const ContractArgs& a, DType t, int nterms, int
 // FP8 (e4m3, one power-of-two scale per 128-k panel of every row) for f32/c32
 // storage: packs A and B into fp8 fragment panels plus scale words (device
 // scratch of fp8_panel_bytes), then multiplies on the scaled fp8 MFMA. Tiles:
-// 256x128 (f32), 128x64 (c32).
+// 256x256 (f32), 128x64 (c32).
 size_t fp8_panel_bytes(const ContractArgs& a, DType t);
 hipError_t launch_contract_fp8(const ContractArgs& a, DType t, void* panels, hipStream_t s);
 // The elementwise (all-batch) strategy: one thread per output element, K

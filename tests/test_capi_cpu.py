@@ -325,8 +325,8 @@ def test_split_k_choice_and_override():
     e = 24
     sp = [1, e, e * e, e ** 3]
     args = (T([e] * 4, sp, [0, 4, 1, 5]), T([e] * 4, sp, [5, 2, 4, 3]), T([e] * 4, sp, [0, 1, 2, 3]))
-    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.32x32x16.dma16.kgroup4"
-    assert make_plan(tp.DType.F64, *args).report()["tile"] == 5
+    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.32x32x16.regs16.kgroup8"
+    assert make_plan(tp.DType.F64, *args).report()["tile"] == 6
     assert make_plan(tp.DType.F32, *args).report()["kernel"] == "f32.mfma_f32.64x64x16.dma16.splitk3"
     assert ".splitk" not in make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=1)).report()["kernel"]
     assert make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=5)).report()["kernel"].endswith(".splitk5")
@@ -413,7 +413,7 @@ def test_plan_scatters_are_the_reduced_matrix_view():
     assert sc["ak"] == tp.build_scatter([e, e], [e ** 3, e])
     # runs of 24 along A's M and B's K: regular with stride 1 at the f64 run of 2
     assert set(tp.build_block_scatter(sc["am"], 2)) == {1} and set(tp.build_block_scatter(sc["bk"], 2)) == {1}
-    assert ".dma16" in p.report()["kernel"]
+    assert ".regs16" in p.report()["kernel"]  # 16-byte run loads
     # one odd offset breaks the runs: A with a leading dimension of 25 (odd row starts)
     q = make_plan(tp.DType.F64, T([6, 8], [1, 7], [2, 0]), T([6, 4], [1, 6], [2, 1]), T([8, 4], [1, 8], [0, 1]))
     assert q.report()["a_k_major"] == 1 and ".dma16" not in q.report()["kernel"]

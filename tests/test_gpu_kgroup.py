@@ -1,9 +1,13 @@
-"""GPU parity of the K-group 32x32 tile (tile id 5): four waves each sum every
-fourth K slab of one 32 x 32 output tile and add their partial tiles in LDS in
-a fixed order. The planner takes it instead of split K for real products
-with a small output and K up to 1024. Checked against the FP64 label oracle over every dtype and complex method,
-both gather orientations of A and B, 16-byte-run and 4-byte gathers, ragged
-M, N and K, conj, alpha and beta * C, batch entries, repeated calls (bitwise
+"""GPU parity of the K-group 32x32 tiles. Tile id 5 (LDS-DMA): four waves
+each sum every fourth K slab of one 32 x 32 output tile and add their partial
+tiles in LDS in a fixed order. Tile id 6 (f64 in 16-byte runs): eight waves,
+each over every eighth slab, with operands loaded straight into registers
+(K-major pairs feed two MFMA k steps, M/N-major pairs the two row blocks).
+The planner takes them instead of split K for real products with a small
+output and K up to 1024. Checked against the FP64 label oracle over every
+dtype and complex method, both gather orientations of A and B, 16-byte-run
+and 4-byte gathers, ragged M, N and K, conj, alpha and beta * C, batch
+entries, K shorter than one round of the groups, repeated calls (bitwise
 equal: the sum order is fixed), and the planner's choice of the tile."""
 import numpy as np
 import pytest
@@ -96,6 +100,8 @@ def test_planner_takes_the_kgroup_tile_for_small_outputs_with_long_k(gpu, dtype)
     kern = plan.report()["kernel"]
     if dtype.is_complex:
         assert ".splitk" in kern, kern
+    elif dtype == tp.DType.F64:
+        assert ".regs16.kgroup8" in kern and ".splitk" not in kern, kern
     else:
         assert ".kgroup4" in kern and ".splitk" not in kern, kern
     av, bv = values(dtype, a, 71), values(dtype, b, 72)
@@ -112,3 +118,50 @@ def test_planner_takes_the_kgroup_tile_for_small_outputs_with_long_k(gpu, dtype)
         outs.append(td.cpu().numpy())
     np.testing.assert_array_equal(outs[0], outs[1])
     np.testing.assert_array_equal(outs[0], got)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("majors", ["kk", "km", "mk", "mm"])
+@pytest.mark.parametrize("runs", [True, False], ids=["runs", "no_runs"])
+def test_register_direct_tile_against_the_oracle(gpu, majors, runs):
+    """Tile 6 on f64: every gather orientation, ragged M, N (not multiples of
+    32) and K (not a multiple of 16), C row-major while D is column-major,
+    alpha and beta; without 16-byte runs (odd extents) the plan runs as the
+    K-group or register-staged 32x32 tile."""
+    a_km, b_km = majors[0] == "k", majors[1] == "k"
+    M, Kx, N = (202, 590, 130) if runs else (201, 589, 131)
+    a, b, d = _operands(M, Kx, N, a_km, b_km, conj=False)
+    c = T([M, N], [N, 1], [I, J])
+    av, bv, cv = values(tp.DType.F64, a, 81), values(tp.DType.F64, b, 82), values(tp.DType.F64, c, 83)
+    dv = np.full(d.storage(), np.nan)
+    got, want, plan = run(tp.DType.F64, a, av, b, bv, d, dv, alpha=0.75, beta=-0.5, c=c, cv=cv,
+                          config=tp.PlanConfig(tile=6))
+    kern = plan.report()["kernel"]
+    assert plan.report()["tile"] == 6
+    assert (".regs16.kgroup8" in kern) == runs, kern
+    check(tp.DType.F64, got, want)
+
+
+@pytest.mark.gpu
+def test_register_direct_tile_with_batch_entries_short_k_and_repeats(gpu):
+    """Batch entries in one launch, K from two pairs (one slab, partly past K)
+    to one slab past a round of the eight groups (groups past K load zeros
+    from the sentinel), and two calls bitwise equal."""
+    import torch
+
+    for Kx in (4, 16, 94, 144):
+        a = T([40, Kx, 3], [1, 40, 40 * Kx], [I, K, H])
+        b = T([Kx, 36, 3], [1, Kx, 36 * Kx], [K, J, H])
+        d = T([40, 36, 3], [1, 40, 40 * 36], [I, J, H])
+        av, bv = values(tp.DType.F64, a, 91 + Kx), values(tp.DType.F64, b, 92 + Kx)
+        dv = np.full(d.storage(), np.nan)
+        plan = make_plan(tp.DType.F64, a, b, d, config=tp.PlanConfig(tile=6))
+        assert ".regs16.kgroup8" in plan.report()["kernel"], plan.report()
+        got, want, _ = run(tp.DType.F64, a, av, b, bv, d, dv, alpha=1.5, plan=plan)
+        check(tp.DType.F64, got, want)
+        dev = torch.device("cuda:0")
+        ta, tb = torch.from_numpy(av).to(dev), torch.from_numpy(bv).to(dev)
+        td = torch.full((d.storage(),), float("nan"), dtype=torch.float64, device=dev)
+        plan.execute_raw(1.5, ta.data_ptr(), tb.data_ptr(), 0.0, None, td.data_ptr())
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(td.cpu().numpy(), got)
