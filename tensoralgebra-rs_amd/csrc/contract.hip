@@ -600,6 +600,7 @@ const TileCfg* tile_cfg(int id) {
       {TILE_128x128, 128, 128, 16, 512, "TILE_128x128"},  // c64: 1024 threads
       {TILE_32x32K, 32, 32, 16, 256, "TILE_32x32K"},
       {TILE_32x32R, 32, 32, 16, 512, "TILE_32x32R"},
+      {TILE_32x48R, 32, 48, 16, 512, "TILE_32x48R"},
       {STILE_128x128, 128, 128, 32, 256, "STILE_128x128"},
       {STILE_64x64, 64, 64, 32, 256, "STILE_64x64"},
       {STILE_256x128, 256, 128, 32, 512, "STILE_256x128"},
@@ -624,10 +625,11 @@ hipError_t launch_contract(const ContractArgs& a, DType t, int method, int tile_
   // as the LDS-DMA K-group tile otherwise), else as the register-staged
   // 32x32 tile; the DMA-only 128x128 tile falls back to the register-staged
   // 64x64 kernel.
-  if (tile_id == TILE_32x32R) {
+  if (tile_id == TILE_32x32R || tile_id == TILE_32x48R) {
+    const bool wide = tile_id == TILE_32x48R && a.b_kmajor;
     if (a.ksplit <= 1 && direct_supported(a.off32, a.ab_vec, t) &&
         ((a.M + 31) / 32) * ((a.N + 31) / 32) * a.H <= max_grid(512))
-      return launch_contract_direct(a, t, s);
+      return launch_contract_direct(a, t, wide, s);
     tile_id = TILE_32x32K;
   }
   if (tile_id == TILE_32x32K) {

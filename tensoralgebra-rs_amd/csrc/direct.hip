@@ -46,21 +46,24 @@ __device__ inline v2d load16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
   return __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
 }
 
-template <bool AKM_, bool BKM_, int KG_, int DEPTH_>
+// NB: column blocks of 16 (BN = 16 NB); three only with B gathered along K
+// (an N-major pair feeds two blocks). MINW: waves per SIMD the registers must
+// allow (KG / 2: two workgroups per CU; KG / 4: one).
+template <bool AKM_, bool BKM_, int KG_, int DEPTH_, int NB_, int MINW_>
 struct RCfg {
   static constexpr bool AKM = AKM_, BKM = BKM_;
-  static constexpr int KG = KG_, DEPTH = DEPTH_;
-  static constexpr int BM = 32, BN = 32, BK = 16, LDD = BN + 1;
+  static constexpr int KG = KG_, DEPTH = DEPTH_, NB = NB_, MINW = MINW_;
+  static constexpr int BM = 32, BN = 16 * NB, BK = 16, LDD = BN + 1;
   static constexpr int NT = 64 * KG;
-  // Waves per SIMD the registers must allow: two workgroups per CU.
-  static constexpr int MINW = KG / 2;
+  static_assert(NB == 2 || (NB == 3 && BKM), "three column blocks need B gathered along K");
 };
 
-// One slab's operands as a lane holds them: four 16-byte loads of A and four
-// of B (K-major: [2 i + p] = rows of block i, k pair p; M/N-major: [s] = the
-// row pair of step s).
+// One slab's operands as a lane holds them: four 16-byte loads of A and
+// 2 NB of B (K-major: [2 i + p] = rows of block i, k pair p; M/N-major:
+// [s] = the row pair of step s).
+template <int NB>
 struct Slab {
-  v2d a[4], b[4];
+  v2d a[4], b[2 * NB];
 };
 
 template <class C>
@@ -81,13 +84,17 @@ __global__ __launch_bounds__(C::NT, C::MINW) void k_contract_direct(ContractArgs
   // Row byte offsets: K-major, the lane's row of each block; M/N-major, the
   // first row of the lane's pair (the extent is even, so a pair is wholly in
   // or out of range).
-  uint32_t a_row[2], b_row[2];
+  constexpr int NB = C::NB;
+  uint32_t a_row[2], b_row[NB];
 #pragma unroll
   for (int i = 0; i < 2; i++) {
     const int64_t m = C::AKM ? m0 + 16 * i + lr : m0 + 2 * lr;
-    const int64_t n = C::BKM ? n0 + 16 * i + lr : n0 + 2 * lr;
     a_row[i] = m < M ? (uint32_t)(p.am[m] * 8) : dev::ROW_OOB;
-    b_row[i] = n < N ? (uint32_t)(p.bn[n] * 8) : dev::ROW_OOB;
+  }
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    const int64_t n = C::BKM ? n0 + 16 * j + lr : n0 + 2 * lr;
+    b_row[j] = n < N ? (uint32_t)(p.bn[n] * 8) : dev::ROW_OOB;
   }
 
   // Entry base + 2 q of a slab's K offsets for lane group q, as masks (a
@@ -111,35 +118,37 @@ __global__ __launch_bounds__(C::NT, C::MINW) void k_contract_direct(ContractArgs
   // The loads of a slab from its K offsets; a slab past the group's last
   // loads zeros from the out-of-range sentinel (no memory access), so every
   // path through the loop issues the same loads and the waits stay exact.
-  auto issue = [&](const KOff& ko, bool valid, Slab& sl) {
+  auto issue = [&](const KOff& ko, bool valid, Slab<NB>& sl) {
     const uint32_t oob = valid ? 0u : dev::K_OOB;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
       if constexpr (C::AKM) sl.a[t] = load16(rsA, a_row[t >> 1] + (pick(ko.a, 8 * (t & 1)) | oob));
       else sl.a[t] = load16(rsA, a_row[0] + (pick(ko.a, 8 * (t >> 1) + (t & 1)) | oob));
+    }
+#pragma unroll
+    for (int t = 0; t < 2 * NB; t++) {
       if constexpr (C::BKM) sl.b[t] = load16(rsB, b_row[t >> 1] + (pick(ko.b, 8 * (t & 1)) | oob));
       else sl.b[t] = load16(rsB, b_row[0] + (pick(ko.b, 8 * (t >> 1) + (t & 1)) | oob));
     }
   };
 
-  dev::v4d acc[2][2];
+  dev::v4d acc[2][NB];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = dev::v4d{0.0, 0.0, 0.0, 0.0};
-  auto compute = [&](const Slab& sl) {
+    for (int j = 0; j < NB; j++) acc[i][j] = dev::v4d{0.0, 0.0, 0.0, 0.0};
+  auto compute = [&](const Slab<NB>& sl) {
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-      double av[2], bv[2];
+      double av[2], bv[NB];
 #pragma unroll
-      for (int i = 0; i < 2; i++) {
-        av[i] = C::AKM ? sl.a[2 * i + (s >> 1)][s & 1] : sl.a[s][i];
-        bv[i] = C::BKM ? sl.b[2 * i + (s >> 1)][s & 1] : sl.b[s][i];
-      }
+      for (int i = 0; i < 2; i++) av[i] = C::AKM ? sl.a[2 * i + (s >> 1)][s & 1] : sl.a[s][i];
+#pragma unroll
+      for (int j = 0; j < NB; j++) bv[j] = C::BKM ? sl.b[2 * j + (s >> 1)][s & 1] : sl.b[s][j];
 #pragma unroll
       for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NB; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
     }
   };
 
@@ -151,7 +160,7 @@ __global__ __launch_bounds__(C::NT, C::MINW) void k_contract_direct(ContractArgs
   // exactly the loads still in flight); the last slabs, already loaded, run
   // after it.
   constexpr int RING = DEPTH + 1;
-  Slab buf[RING];
+  Slab<NB> buf[RING];
   KOff ko = koffs(0);
 #pragma unroll
   for (int d = 0; d < DEPTH; d++) {
@@ -179,7 +188,7 @@ __global__ __launch_bounds__(C::NT, C::MINW) void k_contract_direct(ContractArgs
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < NB; j++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int r = lq + 4 * q;  // the output row within the block
@@ -213,23 +222,22 @@ hipError_t launch_rcfg(const ContractArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int KG, int DEPTH>
-hipError_t launch_direct_majors(const ContractArgs& a, hipStream_t s) {
-  if (a.a_kmajor) {
-    if (a.b_kmajor) return launch_rcfg<RCfg<true, true, KG, DEPTH>>(a, s);
-    return launch_rcfg<RCfg<true, false, KG, DEPTH>>(a, s);
-  }
-  if (a.b_kmajor) return launch_rcfg<RCfg<false, true, KG, DEPTH>>(a, s);
-  return launch_rcfg<RCfg<false, false, KG, DEPTH>>(a, s);
+// 32 x 32: two workgroups per CU, one slab in flight per wave; 32 x 48 (B
+// gathered along K): one workgroup per CU, two slabs in flight.
+template <bool AKM>
+hipError_t launch_direct_b(const ContractArgs& a, bool wide, hipStream_t s) {
+  if (wide) return launch_rcfg<RCfg<AKM, true, 8, 2, 3, 2>>(a, s);
+  if (a.b_kmajor) return launch_rcfg<RCfg<AKM, true, 8, 1, 2, 4>>(a, s);
+  return launch_rcfg<RCfg<AKM, false, 8, 1, 2, 4>>(a, s);
 }
 
 }  // namespace
 
 bool direct_supported(bool off32, bool ab_vec, DType t) { return off32 && ab_vec && t == DType::F64; }
 
-hipError_t launch_contract_direct(const ContractArgs& a, DType t, hipStream_t s) {
-  if (a.ksplit > 1 || !direct_supported(a.off32, a.ab_vec, t)) return hipErrorInvalidValue;
-  return launch_direct_majors<8, 1>(a, s);
+hipError_t launch_contract_direct(const ContractArgs& a, DType t, bool wide, hipStream_t s) {
+  if (a.ksplit > 1 || !direct_supported(a.off32, a.ab_vec, t) || (wide && !a.b_kmajor)) return hipErrorInvalidValue;
+  return a.a_kmajor ? launch_direct_b<true>(a, wide, s) : launch_direct_b<false>(a, wide, s);
 }
 
 }  // namespace tp

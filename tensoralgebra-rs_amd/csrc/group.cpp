@@ -177,7 +177,7 @@ bool group_tile_ok(const Group& g, int id) {
   if (id == TILE_128x128)
     return g.all_off32 && dma_supported(true, g.dtype, g.method, id) && g.dtype != DType::C64 &&
            !(g.dtype == DType::C32 && g.method == CPLX_1M);
-  return tile_cfg(id) != nullptr && id < 10 && id != TILE_32x32K && id != TILE_32x32R && !(id == TILE_128x64 && g.dtype == DType::C64);
+  return tile_cfg(id) != nullptr && id < 10 && id != TILE_32x32K && id != TILE_32x32R && id != TILE_32x48R && !(id == TILE_128x64 && g.dtype == DType::C64);
 }
 
 Group* group_of(tprims_group h) {

@@ -23,6 +23,7 @@ enum TileId : int {
   TILE_128x128 = 4, // LDS-DMA kernel only (16 waves of 32x32, or 8 of 64x32: by dtype, schedule, majors)
   TILE_32x32K = 5,  // LDS-DMA kernel only: 4 waves of 32x32, each over every fourth K slab
   TILE_32x32R = 6,  // f64 in 16-byte runs: 8 waves of 32x32, each over every eighth K slab, loading into registers
+  TILE_32x48R = 7,  // the same on 32x48 tiles (B gathered along K)
 };
 // Tile ids of the split-BF16 kernels (16x16x32 bf16 MFMA family).
 enum SplitTileId : int {
@@ -53,10 +54,10 @@ hipError_t launch_contract_dma(const ContractArgs& a, DType t, int method, int t
 bool kgroup_supported(bool off32, bool ab_vec, bool a_kmajor, bool b_kmajor, DType t);
 hipError_t launch_contract_kgroup(const ContractArgs& a, DType t, int method, hipStream_t s);
 // The 32x32 tile on eight K groups with operands loaded straight into
-// registers (TILE_32x32R): f64, 32-bit gather offsets, A and B in 16-byte
-// runs (any majors), one K range.
+// registers (TILE_32x32R; wide: 32x48, TILE_32x48R, B gathered along K):
+// f64, 32-bit gather offsets, A and B in 16-byte runs, one K range.
 bool direct_supported(bool off32, bool ab_vec, DType t);
-hipError_t launch_contract_direct(const ContractArgs& a, DType t, hipStream_t s);
+hipError_t launch_contract_direct(const ContractArgs& a, DType t, bool wide, hipStream_t s);
 // Split-BF16 for f32/c32 storage (nterms 2 or 3): A and B are split once into
 // bf16 fragment panels (device scratch of split_panel_bytes), then multiplied.
 size_t split_panel_bytes(const ContractArgs& a, DType t, int nterms, int tile_id);

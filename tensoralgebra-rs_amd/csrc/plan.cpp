@@ -318,7 +318,11 @@ void configure(Plan* plan, const PlanConfig& cfg_in) {
     const TileCfg* tc = tile_cfg(id);
     return ((p.M + tc->bm - 1) / tc->bm) * ((p.N + tc->bn - 1) / tc->bn) * p.H;
   };
-  bool kgroup_candidate = false;
+  // Split K would run and the K-group tiles may replace it: K-group (LDS-DMA)
+  // where 32x32 tiles give at least 192 workgroups, register-direct (f64 in
+  // 16-byte runs) at any count (202 x 300 x 130: 6.9 against 12.5 us split
+  // three ways; three 96 x 800 x 96 entries 11.9 against 14.1).
+  bool kgroup_candidate = false, direct_candidate = false;
   if (cfg.split_k < 0 || cfg.split_k > 64) fail(ERR_INVALID_ARGUMENT, "split_k must be in [0, 64]");
   if (p.strategy == 0 && cfg.precision == PREC_STORAGE && cfg.split_k != 1) {
     const TileCfg* tc = tile_cfg(p.tile);
@@ -326,13 +330,13 @@ void configure(Plan* plan, const PlanConfig& cfg_in) {
     const int64_t nk = (p.K + tc->bk - 1) / tc->bk;
     int64_t s = 1;
     if (cfg.split_k > 1) s = std::min<int64_t>(cfg.split_k, nk);
-    else if (tiles < 128 && nk >= 16 && p.tile != TILE_32x32K && p.tile != TILE_32x32R)
+    else if (tiles < 128 && nk >= 16 && p.tile != TILE_32x32K && p.tile != TILE_32x32R && p.tile != TILE_32x48R)
       s = std::min<int64_t>(std::min<int64_t>(256 / tiles, nk / 6), 16);
     if (s >= 2) {
       p.ksplit = (int)s;
       p.kslabs = (nk + s - 1) / s;
-      kgroup_candidate = cfg.split_k == 0 && cfg.tile == 0 && !is_complex(pr.dtype) && p.K <= 1024 &&
-                         tiles_of(TILE_32x32K) >= 192;
+      direct_candidate = cfg.split_k == 0 && cfg.tile == 0 && !is_complex(pr.dtype) && p.K <= 1024;
+      kgroup_candidate = direct_candidate && tiles_of(TILE_32x32K) >= 192;
     }
   }
   p.ak32.clear();
@@ -385,9 +389,14 @@ void configure(Plan* plan, const PlanConfig& cfg_in) {
   // f64 in 16-byte runs takes the register-direct K-group tile (f64 24^6:
   // 16.4 against 20.6 us per call on the LDS-DMA one; 576 x 1152 x 576 27.8
   // against 34.2).
-  if (kgroup_candidate && (direct_supported(p.off32, p.ab_vec, pr.dtype) ||
-                           kgroup_supported(p.off32, p.ab_vec, p.a_kmajor, p.b_kmajor, pr.dtype))) {
-    p.tile = direct_supported(p.off32, p.ab_vec, pr.dtype) ? TILE_32x32R : TILE_32x32K;
+  const bool direct = direct_candidate && direct_supported(p.off32, p.ab_vec, pr.dtype);
+  if (direct || (kgroup_candidate && kgroup_supported(p.off32, p.ab_vec, p.a_kmajor, p.b_kmajor, pr.dtype))) {
+    p.tile = direct ? TILE_32x32R : TILE_32x32K;
+    // More 32x32 tiles than CUs put two on some CUs: 32x48 tiles (B gathered
+    // along K), one per CU, when they all fit in one round (f64 24^6: 324
+    // tiles of 32x32, 216 of 32x48).
+    if (p.tile == TILE_32x32R && p.b_kmajor && tiles_of(TILE_32x32R) > 256 && tiles_of(TILE_32x48R) <= 256)
+      p.tile = TILE_32x48R;
     p.ksplit = 1;
     p.kslabs = 0;
   }
@@ -598,10 +607,16 @@ std::string Plan::kernel_name() const {
   // The register-direct K-group tile runs as the LDS-DMA K-group tile where
   // its loads do not fit, and both as the register-staged 32x32 tile where
   // neither does.
-  const bool direct = tile == TILE_32x32R && ksplit <= 1 && direct_supported(off32, ab_vec, prob.dtype);
-  const bool kgroup = (tile == TILE_32x32K || tile == TILE_32x32R) && !direct && ksplit <= 1 &&
+  const bool rtile = tile == TILE_32x32R || tile == TILE_32x48R;
+  const bool direct = rtile && ksplit <= 1 && direct_supported(off32, ab_vec, prob.dtype);
+  const bool kgroup = (tile == TILE_32x32K || rtile) && !direct && ksplit <= 1 &&
                       kgroup_supported(off32, ab_vec, a_kmajor, b_kmajor, prob.dtype);
-  const TileCfg* t = tile_cfg(fallback ? TILE_64x64 : tile == TILE_32x32R && !direct && !kgroup ? TILE_32x32 : tile);
+  // 32x48 runs as 32x32 with B gathered along N.
+  const int shown = fallback                                  ? TILE_64x64
+                    : rtile && !direct && !kgroup             ? TILE_32x32
+                    : rtile && (!direct || !b_kmajor)         ? TILE_32x32K
+                                                              : tile;
+  const TileCfg* t = tile_cfg(shown);
   if (t) s += "." + std::to_string(t->bm) + "x" + std::to_string(t->bn) + "x" + std::to_string(cfg.precision == PREC_FP8 ? 128 : t->bk);
   if (cfg.precision == PREC_STORAGE &&
       ((!fallback && dma_supported(off32, prob.dtype, cfg.complex_method, tile)) || kgroup))

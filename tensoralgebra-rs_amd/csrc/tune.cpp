@@ -98,7 +98,7 @@ void replan(Plan& p, const PlanConfig& cfg) {
 // The tiles a plan of this precision can run (tile_allowed filters per dtype
 // and complex method).
 std::vector<int> tile_candidates(int prec) {
-  if (prec == PREC_STORAGE) return {TILE_128x128, TILE_128x64, TILE_64x64, TILE_32x32, TILE_32x32K, TILE_32x32R};
+  if (prec == PREC_STORAGE) return {TILE_128x128, TILE_128x64, TILE_64x64, TILE_32x32, TILE_32x32K, TILE_32x32R, TILE_32x48R};
   if (prec == PREC_FP8) return {STILE_256x256, STILE_128x64};
   return {STILE_256x128, STILE_128x128, STILE_128x64, STILE_64x64};
 }

@@ -314,19 +314,46 @@ def test_planner_picks_elementwise_for_all_batch_products():
     assert gemm["strategy"] == 0
 
 
+def test_register_direct_tile_choice():
+    """f64 in 16-byte runs with K <= 1024 where split K would run: 32x32 tiles
+    on eight K groups at any tile count, 32x48 when more than 256 tiles of
+    32x32 become at most 256 of 32x48 (B gathered along K); split K beyond
+    K = 1024 and for f32; an explicit tile or split wins."""
+    import tprims_amd as tp
+    from harness import T, make_plan
+
+    def plan(M, K, N, dt=tp.DType.F64, **cfg):
+        a, b, d = T([M, K], [1, M], [0, 2]), T([K, N], [1, K], [2, 1]), T([M, N], [1, M], [0, 1])
+        return make_plan(dt, a, b, d, config=tp.PlanConfig(**cfg) if cfg else None).report()
+
+    assert plan(202, 300, 130)["tile"] == 6  # 35 tiles
+    assert plan(512, 512, 512)["tile"] == 6  # 256 tiles: one per CU already
+    assert plan(576, 576, 576)["tile"] == 7  # 324 -> 216
+    assert plan(640, 1024, 640)["tile"] == 6  # 400 -> 280: 32x48 would need a second round
+    assert ".splitk" in plan(384, 2048, 384)["kernel"]
+    assert ".splitk" in plan(202, 300, 130, tp.DType.F32)["kernel"]
+    assert plan(576, 576, 576, tile=6)["kernel"] == "f64.mfma_f64.32x32x16.regs16.kgroup8"
+    assert plan(576, 576, 576, split_k=3)["kernel"].endswith(".splitk3")
+    # 32x48 with B gathered along N runs (and reports) the 32x32 form
+    a, b, d = T([576, 576], [1, 576], [0, 2]), T([576, 576], [576, 1], [2, 1]), T([576, 576], [1, 576], [0, 1])
+    r = make_plan(tp.DType.F64, a, b, d, config=tp.PlanConfig(tile=7)).report()
+    assert r["b_k_major"] == 0 and r["kernel"] == "f64.mfma_f64.32x32x16.regs16.kgroup8", r
+
+
 def test_split_k_choice_and_override():
-    """Few 64x64 output tiles with a long K: the planner takes the K-group
-    32x32 tile where 32x32 tiles give at least 192 workgroups (the f64 24^6
-    contraction: 324) and its gathers allow, else splits K (the f32 form of
-    the same contraction, whose A gathers along M); split_k = 1 keeps one K
-    range, n > 1 asks for n; out-of-range values are refused."""
+    """Few 64x64 output tiles with a long K: the planner takes the
+    register-direct K-group tile for f64 in 16-byte runs (on 32x48 tiles for
+    the f64 24^6 contraction, whose 324 tiles of 32x32 would put two on some
+    CUs), else splits K (the f32 form of the same contraction, whose A
+    gathers along M); split_k = 1 keeps one K range, n > 1 asks for n;
+    out-of-range values are refused."""
     import tprims_amd as tp
     from harness import T, make_plan
     e = 24
     sp = [1, e, e * e, e ** 3]
     args = (T([e] * 4, sp, [0, 4, 1, 5]), T([e] * 4, sp, [5, 2, 4, 3]), T([e] * 4, sp, [0, 1, 2, 3]))
-    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.32x32x16.regs16.kgroup8"
-    assert make_plan(tp.DType.F64, *args).report()["tile"] == 6
+    assert make_plan(tp.DType.F64, *args).report()["kernel"] == "f64.mfma_f64.32x48x16.regs16.kgroup8"
+    assert make_plan(tp.DType.F64, *args).report()["tile"] == 7
     assert make_plan(tp.DType.F32, *args).report()["kernel"] == "f32.mfma_f32.64x64x16.dma16.splitk3"
     assert ".splitk" not in make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=1)).report()["kernel"]
     assert make_plan(tp.DType.F64, *args, config=tp.PlanConfig(split_k=5)).report()["kernel"].endswith(".splitk5")

@@ -43,8 +43,12 @@ def _items(torch, dtype, a, b, d, n, seed, gap=3):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
-@pytest.mark.parametrize("shape", ["gemm", "batched_strided", "split_k", "elementwise", "negative_strides"])
+@pytest.mark.parametrize("shape", ["gemm", "batched_strided", "split_k", "long_k", "elementwise", "negative_strides"])
 def test_batched_items_match_the_oracle(gpu, dtype, shape):
+    """Items of one plan in one fused launch against the oracle: a GEMM, a
+    padded batched layout with conj, split K (asked for), the planner's own
+    choice for a small output with a long K (the register-direct K-group
+    tile for f64), the elementwise strategy and negative strides."""
     import torch
 
     if shape == "gemm":
@@ -60,16 +64,18 @@ def test_batched_items_match_the_oracle(gpu, dtype, shape):
         b = T([30, 24], [24, 1], [2, 1])
         d = T([20, 24], [-24, 1], [0, 1])
         n = 6
-    elif shape == "split_k":
+    elif shape in ("split_k", "long_k"):
         a, b, d = T([40, 900], [1, 40], [0, 2]), T([900, 24], [1, 900], [2, 1]), T([40, 24], [1, 40], [0, 1])
         n = 3
     else:
         H = 1100
         a, b, d = T([2, 3, H], [1, 2, 6], [0, 2, 3]), T([3, 2, H], [1, 3, 6], [2, 1, 3]), T([2, 2, H], [1, 2, 4], [0, 1, 3])
         n = 4
-    plan = make_plan(dtype, a, b, d, d)
+    plan = make_plan(dtype, a, b, d, d, tp.PlanConfig(split_k=3) if shape == "split_k" else None)
     if shape == "split_k":
         assert ".splitk" in plan.report()["kernel"]
+    if shape == "long_k" and dtype == tp.DType.F64:
+        assert ".regs16.kgroup8" in plan.report()["kernel"], plan.report()
     if shape == "elementwise":
         assert plan.report()["strategy"] == 3
     alpha, beta = scalar(dtype, 0.75, -0.25), scalar(dtype, -0.5, 0.5)

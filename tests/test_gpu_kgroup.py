@@ -121,13 +121,15 @@ def test_planner_takes_the_kgroup_tile_for_small_outputs_with_long_k(gpu, dtype)
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tile", [6, 7])
 @pytest.mark.parametrize("majors", ["kk", "km", "mk", "mm"])
 @pytest.mark.parametrize("runs", [True, False], ids=["runs", "no_runs"])
-def test_register_direct_tile_against_the_oracle(gpu, majors, runs):
-    """Tile 6 on f64: every gather orientation, ragged M, N (not multiples of
-    32) and K (not a multiple of 16), C row-major while D is column-major,
-    alpha and beta; without 16-byte runs (odd extents) the plan runs as the
-    K-group or register-staged 32x32 tile."""
+def test_register_direct_tile_against_the_oracle(gpu, tile, majors, runs):
+    """Tiles 6 (32x32) and 7 (32x48; 32x32 where B gathers along N) on f64:
+    every gather orientation, ragged M, N (not multiples of the tile) and K
+    (not a multiple of 16), C row-major while D is column-major, alpha and
+    beta; without 16-byte runs (odd extents) the plan runs as the K-group or
+    register-staged 32x32 tile."""
     a_km, b_km = majors[0] == "k", majors[1] == "k"
     M, Kx, N = (202, 590, 130) if runs else (201, 589, 131)
     a, b, d = _operands(M, Kx, N, a_km, b_km, conj=False)
@@ -135,10 +137,11 @@ def test_register_direct_tile_against_the_oracle(gpu, majors, runs):
     av, bv, cv = values(tp.DType.F64, a, 81), values(tp.DType.F64, b, 82), values(tp.DType.F64, c, 83)
     dv = np.full(d.storage(), np.nan)
     got, want, plan = run(tp.DType.F64, a, av, b, bv, d, dv, alpha=0.75, beta=-0.5, c=c, cv=cv,
-                          config=tp.PlanConfig(tile=6))
+                          config=tp.PlanConfig(tile=tile))
     kern = plan.report()["kernel"]
-    assert plan.report()["tile"] == 6
+    assert plan.report()["tile"] == tile
     assert (".regs16.kgroup8" in kern) == runs, kern
+    assert ("32x48" in kern) == (runs and tile == 7 and b_km), kern
     check(tp.DType.F64, got, want)
 
 
@@ -149,13 +152,13 @@ def test_register_direct_tile_with_batch_entries_short_k_and_repeats(gpu):
     from the sentinel), and two calls bitwise equal."""
     import torch
 
-    for Kx in (4, 16, 94, 144):
+    for tile, Kx in ((6, 4), (6, 16), (6, 94), (6, 144), (7, 4), (7, 94), (7, 144)):
         a = T([40, Kx, 3], [1, 40, 40 * Kx], [I, K, H])
         b = T([Kx, 36, 3], [1, Kx, 36 * Kx], [K, J, H])
         d = T([40, 36, 3], [1, 40, 40 * 36], [I, J, H])
         av, bv = values(tp.DType.F64, a, 91 + Kx), values(tp.DType.F64, b, 92 + Kx)
         dv = np.full(d.storage(), np.nan)
-        plan = make_plan(tp.DType.F64, a, b, d, config=tp.PlanConfig(tile=6))
+        plan = make_plan(tp.DType.F64, a, b, d, config=tp.PlanConfig(tile=tile))
         assert ".regs16.kgroup8" in plan.report()["kernel"], plan.report()
         got, want, _ = run(tp.DType.F64, a, av, b, bv, d, dv, alpha=1.5, plan=plan)
         check(tp.DType.F64, got, want)

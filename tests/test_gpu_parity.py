@@ -186,9 +186,10 @@ def test_split_k(gpu, dtype, shape, d_major):
         if dtype.is_complex:
             v = v + 1j * rng.uniform(-1, 1, t.storage())
         return v.astype(NP[dtype])
-    # 576^3: the planner takes the K-group 32x32 tile where its gathers allow
-    # (f64, c32), so three K ranges are asked for.
-    cfg = tp.PlanConfig(split_k=3) if M == 576 else None
+    # K up to 1024: the planner takes a K-group 32x32 tile where its gathers
+    # allow (f64 at any tile count; c32 with 324 tiles), so three K ranges
+    # are asked for.
+    cfg = tp.PlanConfig(split_k=3) if K <= 1024 else None
     got, want, plan = run(dtype, a, vals(a), b, vals(b), d, np.zeros(d.storage(), NP[dtype]),
                           alpha=1.5, beta=-0.5, c=d, cv=vals(d), config=cfg)
     assert ".splitk" in plan.report()["kernel"]

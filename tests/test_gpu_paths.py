@@ -80,7 +80,7 @@ def test_f64_24_contraction_through_the_tapp_abi(gpu, variant):
     elif variant == "split_k3":
         assert ".dma16.splitk3" in kern, kern
     else:
-        assert ".regs16.kgroup8" in kern, kern  # the planner's choice: 16-byte runs, eight K groups
+        assert "32x48x16.regs16.kgroup8" in kern, kern  # the planner's choice: 16-byte runs, eight K groups
     rng = np.random.default_rng(24)
     av, bv = rng.uniform(-1, 1, e ** 4), rng.uniform(-1, 1, e ** 4)
     dv = np.full(e ** 4, np.nan)  # beta = 0: D is written, never read

@@ -32,7 +32,7 @@ def test_cached_entry_drives_the_plan(clean_cache, tmp_path):
     a, b, d = _f64_24()
     plan = make_plan(tp.DType.F64, a, b, d)
     r = plan.report()
-    assert r["source"] == 0 and r["tile"] == 6  # the planner's model: the register-direct K-group tile
+    assert r["source"] == 0 and r["tile"] == 7  # the planner's model: the register-direct K-group tile, 32x48
     key = plan.tuning_key()
     assert key.startswith("dt=") and ";m=" in key and " " not in key
     f = tmp_path / "cache.txt"
