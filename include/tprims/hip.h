@@ -49,7 +49,9 @@ typedef struct {
   int32_t precision;      /* TPRIMS_PREC_* */
   int32_t tile;           /* 0: chosen by the planner's tuner; else a tile id (see report): storage
                              1 64x64, 2 32x32, 3 128x64, 4 128x128 (LDS-DMA), 5 32x32 on four K groups
-                             (LDS-DMA; runs as 2 where its gathers do not fit); split-BF16 11 128x128,
+                             (LDS-DMA; runs as 2 where its gathers do not fit), 6 32x32 and 7 32x48 on
+                             eight K groups loading into registers (f64 in 16-byte runs; 7 needs B
+                             gathered along K, else runs as 6; both run as 5 otherwise); split-BF16 11 128x128,
                              12 64x64, 13 256x128, 14 128x64; FP8 14 128x64 (c32), 15 256x256 (f32) */
   int32_t split_k;        /* 0: chosen by the planner; 1: never split K; n > 1: n K ranges
                              (storage precision; partial tiles reduced in split order) */
