@@ -15,8 +15,10 @@
 // row l % 16 and 32 bytes of fp8; A and B lanes pair byte for byte, and the
 // lane's scale register (its low byte) scales one 32-k quarter of the 128-k
 // step. With one scale per 128-k panel every quarter of a row carries the
-// same value, so the layout of k inside the 32 bytes does not matter; the
-// packer gives lane l the k range 32 (l / 16) .. + 31 of its row.
+// same value, so any assignment of a panel's 128 k to (quarter, byte) works
+// when A and B share it. Every packer uses kslot(q, b) = 16 (b / 4) + 4 q +
+// b % 4 (byte b of quarter q = lane / 16): the four lanes of a row then read
+// 64 contiguous bytes of a K-major row per 16-byte load.
 //
 // Two stages, as in the split-BF16 path:
 //  1. k_pack_fp8: one wave per (batch h, 128-k block, 16-row block): each lane
@@ -47,6 +49,9 @@ typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 constexpr int F8_FRAG = 2048;  // bytes per fragment (16 rows x 128 k)
 constexpr int F8_KBLK = 128;
 constexpr float F8_MAX = 448.0f;
+
+// The k within a 128-k panel that byte b (0..31) of quarter q (0..3) holds.
+__device__ inline int kslot(int q, int b) { return 16 * (b >> 2) + 4 * q + (b & 3); }
 
 struct F8PackArgs {
   const void* X;
@@ -89,42 +94,43 @@ __global__ __launch_bounds__(256) void k_pack_fp8(F8PackArgs p) {
   const int64_t h = bid / p.KB;
   const int64_t rb = rg * 4 + (threadIdx.x >> 6);
   const int64_t r = rb * 16 + (lane & 15);
-  const int64_t k0 = kb * F8_KBLK + 32 * (lane >> 4);
+  const int lq = lane >> 4;
+  const int64_t k0 = kb * F8_KBLK;
   const bool rok = r < p.R;
-  // Row 0 and k 0 have offset 0: X (and X + 0..7 under VEC) is a valid address.
+  // Row 0 and k 0 have offset 0: X (and X + 0..3 under VEC) is a valid address.
   const S* __restrict__ X = (const S*)p.X + p.hv[h] + (rok ? p.rv[r] : 0);
   float re[32], im[32];
 #pragma unroll
-  for (int c8 = 0; c8 < 4; c8++) {
+  for (int t = 0; t < 8; t++) {
+    // Bytes 4 t .. 4 t + 3: k = k0 + 16 t + 4 lq + 0..3.
+    const int64_t kt = k0 + kslot(lq, 4 * t);
     if constexpr (VEC) {
-      const bool ok = rok && k0 + 8 * c8 < p.K;  // K % 8 == 0: 8 k in or out together
-      const v4f* src = (const v4f*)(X + (ok ? k0 + 8 * c8 : 0));
+      const bool ok = rok && kt < p.K;  // K % 8 == 0: the 4 k in or out together
+      const v4f* src = (const v4f*)(X + (ok ? kt : 0));
       if constexpr (!CPLX) {
+        const v4f x = src[0];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          re[4 * t + j] = ok ? x[j] : 0.0f;
+          im[4 * t + j] = 0.0f;
+        }
+      } else {
         const v4f x0 = src[0], x1 = src[1];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          re[8 * c8 + j] = ok ? x0[j] : 0.0f;
-          re[8 * c8 + 4 + j] = ok ? x1[j] : 0.0f;
-          im[8 * c8 + j] = im[8 * c8 + 4 + j] = 0.0f;
-        }
-      } else {
-        v4f x[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) x[q] = src[q];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          re[8 * c8 + j] = ok ? x[j >> 1][2 * (j & 1)] : 0.0f;
-          im[8 * c8 + j] = ok ? x[j >> 1][2 * (j & 1) + 1] : 0.0f;
+          const v4f& x = j < 2 ? x0 : x1;
+          re[4 * t + j] = ok ? x[2 * (j & 1)] : 0.0f;
+          im[4 * t + j] = ok ? x[2 * (j & 1) + 1] : 0.0f;
         }
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int64_t k = k0 + 8 * c8 + j;
+      for (int j = 0; j < 4; j++) {
+        const int64_t k = kt + j;
         const bool ok = rok && k < p.K;
         const S v = X[ok ? p.kv[k] : 0];
-        re[8 * c8 + j] = ok ? St::re(v) : 0.0f;
-        im[8 * c8 + j] = ok ? St::im(v) : 0.0f;
+        re[4 * t + j] = ok ? St::re(v) : 0.0f;
+        im[4 * t + j] = ok ? St::im(v) : 0.0f;
       }
     }
   }
@@ -196,13 +202,16 @@ __global__ __launch_bounds__(256) void k_pack_fp8_lds(F8PackArgs p) {
       if constexpr (CPLX) tim[rr * LDW + kl] = ok ? St::im(v) : 0.0f;
     }
     __syncthreads();
-    if (lk == ch) {
+    // This chunk holds bytes 8 ch .. 8 ch + 7 of every lane: k = 32 ch + 16 tt
+    // + 4 lk + j within the panel.
 #pragma unroll
-      for (int j = 0; j < 32; j++) {
-        re[j] = tre[rl * LDW + j];
-        im[j] = CPLX ? tim[rl * LDW + j] : 0.0f;
+    for (int tt = 0; tt < 2; tt++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int b = 4 * (2 * ch + tt) + j, kl = 16 * tt + 4 * lk + j;
+        re[b] = tre[rl * LDW + kl];
+        im[b] = CPLX ? tim[rl * LDW + kl] : 0.0f;
       }
-    }
   }
   uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rb) * COMPS * F8_FRAG + lane * 16;
 #pragma unroll
@@ -249,13 +258,13 @@ __global__ __launch_bounds__(256) void k_pack_fp8_rows(F8PackArgs p) {
   const int64_t h = bid / p.KB;
   const int64_t r = rg * 64 + lane;
   const bool rok = r < p.R;
-  const int64_t k0 = kb * F8_KBLK + 32 * q;
+  const int64_t k0 = kb * F8_KBLK;
   // Row 0 and k 0 have offset 0: out-of-range lanes read X[0] and store zero.
   const S* __restrict__ X = (const S*)p.X + p.hv[h] + (rok ? p.rv[r] : 0);
   float re[32], im[32];
 #pragma unroll
   for (int j = 0; j < 32; j++) {
-    const int64_t k = k0 + j;
+    const int64_t k = k0 + kslot(q, j);
     const bool ok = rok && k < p.K;
     const S v = X[k < p.K ? p.kv[k] : 0];
     re[j] = ok ? St::re(v) : 0.0f;
@@ -293,13 +302,20 @@ __global__ __launch_bounds__(256) void k_pack_fp8_rows(F8PackArgs p) {
 // Real operands whose rows come in 16-byte runs (4 consecutive f32 at
 // 16-byte aligned offsets): each lane loads 4 rows at a time, so a load
 // instruction reads 1 KiB (256 rows at one k); the workgroup's four waves
-// take the same 256 rows and one 32-k quarter each and combine the rows'
-// panel maxima through LDS, and each lane stores its four rows' 32-byte
-// quarters into their fragment slots. Rows past the padded panel (RB * 16)
-// are not stored.
+// take the same 256 rows and 32 consecutive k each (K offsets by scalar
+// loads: the range is uniform across the wave) and combine the rows' panel
+// maxima through LDS. The workgroup's output is 16 consecutive fragments (32
+// KiB): each lane puts its four rows' bytes into their fragment slots in LDS
+// (8 bytes per quarter, as kslot places the wave's k; fragment pitch 2064
+// bytes, so the 16 fragments a store instruction touches start on different
+// banks), then the workgroup writes the block out as whole 4 KiB rows.
+// Fragments past the padded panel (RB row blocks) are not written.
+constexpr int F8_LPITCH = F8_FRAG + 16;
+
 __global__ __launch_bounds__(256) void k_pack_fp8_rows4(F8PackArgs p) {
   __shared__ float red[4][256];
-  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  __shared__ __attribute__((aligned(16))) uint8_t frag[16 * F8_LPITCH];
+  const int lane = threadIdx.x & 63, q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t rgroups = (p.RB + 15) / 16;
   int64_t bid = blockIdx.x;
   const int64_t rg = bid % rgroups;
@@ -308,7 +324,7 @@ __global__ __launch_bounds__(256) void k_pack_fp8_rows4(F8PackArgs p) {
   const int64_t h = bid / p.KB;
   const int64_t r0 = rg * 256 + 4 * lane;  // the lane's first row
   const bool rok = r0 < p.R;               // R is a multiple of 4: all four rows or none
-  const int64_t k0 = kb * F8_KBLK + 32 * q;
+  const int64_t k0 = kb * F8_KBLK + 32 * q;  // wave q loads k0 .. k0 + 31
   const float* __restrict__ X = (const float*)p.X + p.hv[h] + (rok ? p.rv[r0] : 0);
   v4f v[32];
 #pragma unroll
@@ -329,21 +345,36 @@ __global__ __launch_bounds__(256) void k_pack_fp8_rows4(F8PackArgs p) {
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int rl = 4 * lane + i;  // row within the group of 256
-    const int64_t r = rg * 256 + rl;
-    if (r >= p.RB * 16) continue;
     const float m = fmaxf(fmaxf(red[0][rl], red[1][rl]), fmaxf(red[2][rl], red[3][rl]));
     const int e = panel_exponent(m);
     const float inv = ldexpf(1.0f, -e);
-    int w[8];
+    // The wave's 32 k are bytes 8 q .. 8 q + 7 of every quarter c (k 4 c +
+    // 16 u + 0..3 is byte 4 (2 q + u) + 0..3 of quarter c): one 8-byte piece
+    // per quarter, in half q / 2 of the slot, at 8 (q % 2).
 #pragma unroll
-    for (int d = 0; d < 8; d++) {
-      int x = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * d][i] * inv, v[4 * d + 1][i] * inv, 0, false);
-      w[d] = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * d + 2][i] * inv, v[4 * d + 3][i] * inv, x, true);
+    for (int c = 0; c < 4; c++) {
+      int w2[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const v4f* x = v + 4 * c + 16 * u;
+        int y = __builtin_amdgcn_cvt_pk_fp8_f32(x[0][i] * inv, x[1][i] * inv, 0, false);
+        w2[u] = __builtin_amdgcn_cvt_pk_fp8_f32(x[2][i] * inv, x[3][i] * inv, y, true);
+      }
+      uint8_t* f = frag + (rl / 16) * F8_LPITCH + (q >> 1) * 1024 + (c * 16 + rl % 16) * 16 + 8 * (q & 1);
+      *(uint2*)f = uint2{(unsigned)w2[0], (unsigned)w2[1]};
     }
-    uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + r / 16) * F8_FRAG + (q * 16 + rl % 16) * 16;
-    *(v4u*)out = v4u{(unsigned)w[0], (unsigned)w[1], (unsigned)w[2], (unsigned)w[3]};
-    *(v4u*)(out + 1024) = v4u{(unsigned)w[4], (unsigned)w[5], (unsigned)w[6], (unsigned)w[7]};
-    if (q == 0) p.scales[(h * p.KB + kb) * p.RB * 16 + r] = 127 + e;
+    const int64_t r = rg * 256 + rl;
+    if (q == 0 && r < p.RB * 16) p.scales[(h * p.KB + kb) * p.RB * 16 + r] = 127 + e;
+  }
+  __syncthreads();
+  // The block: fragments rg * 16 .. + 15 of (h, kb), 128 16-byte pieces each.
+  const int64_t nfr = p.RB - rg * 16 < 16 ? p.RB - rg * 16 : 16;
+  uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rg * 16) * F8_FRAG;
+#pragma unroll
+  for (int t = 0; t < 8; t++) {
+    const int idx = t * 256 + threadIdx.x;  // 16-byte piece of the 32 KiB block
+    const int fr = idx >> 7, pc = idx & 127;
+    if (fr < nfr) *(v4u*)(out + (int64_t)fr * F8_FRAG + pc * 16) = *(const v4u*)(frag + fr * F8_LPITCH + pc * 16);
   }
 }
 
@@ -589,8 +620,8 @@ hipError_t run(const ContractArgs& a, void* panels, hipStream_t s) {
     typedef F8Cfg<true, 128, 64, 32, 32> G;
     hipLaunchKernelGGL(k_gemm_fp8<G>, dim3((unsigned)nwg), dim3(G::NT), 0, s, g);
   } else {
-    // 256 x 256 on four waves of 128 x 128 (one per SIMD): half the LDS
-    // fragment reads per MFMA of 64 x 64 wave tiles.
+    // 256 x 256 on eight waves of 128 x 64 (two per SIMD): fewer LDS
+    // fragment reads per MFMA than 64 x 64 wave tiles.
     typedef F8Cfg<false, 256, 256, 128, 64> G;
     hipLaunchKernelGGL(k_gemm_fp8<G>, dim3((unsigned)nwg), dim3(G::NT), 0, s, g);
   }
