@@ -301,21 +301,21 @@ __global__ __launch_bounds__(256) void k_pack_fp8_rows(F8PackArgs p) {
 
 // Real operands whose rows come in 16-byte runs (4 consecutive f32 at
 // 16-byte aligned offsets): each lane loads 4 rows at a time, so a load
-// instruction reads 1 KiB (256 rows at one k); the workgroup's four waves
-// take the same 256 rows and 32 consecutive k each (K offsets by scalar
+// instruction reads 1 KiB (256 rows at one k); the workgroup's eight waves
+// take the same 256 rows and 16 consecutive k each (K offsets by scalar
 // loads: the range is uniform across the wave) and combine the rows' panel
 // maxima through LDS. The workgroup's output is 16 consecutive fragments (32
 // KiB): each lane puts its four rows' bytes into their fragment slots in LDS
-// (8 bytes per quarter, as kslot places the wave's k; fragment pitch 2064
+// (4 bytes per quarter, as kslot places the wave's k; fragment pitch 2064
 // bytes, so the 16 fragments a store instruction touches start on different
-// banks), then the workgroup writes the block out as whole 4 KiB rows.
+// banks), then the workgroup writes the block out as whole 8 KiB rows.
 // Fragments past the padded panel (RB row blocks) are not written.
 constexpr int F8_LPITCH = F8_FRAG + 16;
 
-__global__ __launch_bounds__(256) void k_pack_fp8_rows4(F8PackArgs p) {
-  __shared__ float red[4][256];
+__global__ __launch_bounds__(512) void k_pack_fp8_rows4(F8PackArgs p) {
+  __shared__ float red[8][256];
   __shared__ __attribute__((aligned(16))) uint8_t frag[16 * F8_LPITCH];
-  const int lane = threadIdx.x & 63, q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t rgroups = (p.RB + 15) / 16;
   int64_t bid = blockIdx.x;
   const int64_t rg = bid % rgroups;
@@ -324,11 +324,11 @@ __global__ __launch_bounds__(256) void k_pack_fp8_rows4(F8PackArgs p) {
   const int64_t h = bid / p.KB;
   const int64_t r0 = rg * 256 + 4 * lane;  // the lane's first row
   const bool rok = r0 < p.R;               // R is a multiple of 4: all four rows or none
-  const int64_t k0 = kb * F8_KBLK + 32 * q;  // wave q loads k0 .. k0 + 31
+  const int64_t k0 = kb * F8_KBLK + 16 * w;  // wave w loads k0 .. k0 + 15
   const float* __restrict__ X = (const float*)p.X + p.hv[h] + (rok ? p.rv[r0] : 0);
-  v4f v[32];
+  v4f v[16];
 #pragma unroll
-  for (int j = 0; j < 32; j++) {
+  for (int j = 0; j < 16; j++) {
     const int64_t k = k0 + j;
     const bool ok = rok && k < p.K;
     const v4f x = *(const v4f*)(X + (k < p.K ? p.kv[k] : 0));
@@ -336,43 +336,40 @@ __global__ __launch_bounds__(256) void k_pack_fp8_rows4(F8PackArgs p) {
   }
   v4f amax = v4f{0, 0, 0, 0};
 #pragma unroll
-  for (int j = 0; j < 32; j++)
+  for (int j = 0; j < 16; j++)
 #pragma unroll
     for (int i = 0; i < 4; i++) amax[i] = fmaxf(amax[i], fabsf(v[j][i]));
 #pragma unroll
-  for (int i = 0; i < 4; i++) red[q][4 * lane + i] = amax[i];
+  for (int i = 0; i < 4; i++) red[w][4 * lane + i] = amax[i];
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int rl = 4 * lane + i;  // row within the group of 256
-    const float m = fmaxf(fmaxf(red[0][rl], red[1][rl]), fmaxf(red[2][rl], red[3][rl]));
+    float m = red[0][rl];
+#pragma unroll
+    for (int g = 1; g < 8; g++) m = fmaxf(m, red[g][rl]);
     const int e = panel_exponent(m);
     const float inv = ldexpf(1.0f, -e);
-    // The wave's 32 k are bytes 8 q .. 8 q + 7 of every quarter c (k 4 c +
-    // 16 u + 0..3 is byte 4 (2 q + u) + 0..3 of quarter c): one 8-byte piece
-    // per quarter, in half q / 2 of the slot, at 8 (q % 2).
+    // The wave's 16 k are byte word w of every quarter c (k 16 w + 4 c + 0..3
+    // is byte 4 w + 0..3 of quarter c): one 4-byte word per quarter, in half
+    // w / 4 of the slot, at 4 (w % 4).
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      int w2[2];
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const v4f* x = v + 4 * c + 16 * u;
-        int y = __builtin_amdgcn_cvt_pk_fp8_f32(x[0][i] * inv, x[1][i] * inv, 0, false);
-        w2[u] = __builtin_amdgcn_cvt_pk_fp8_f32(x[2][i] * inv, x[3][i] * inv, y, true);
-      }
-      uint8_t* f = frag + (rl / 16) * F8_LPITCH + (q >> 1) * 1024 + (c * 16 + rl % 16) * 16 + 8 * (q & 1);
-      *(uint2*)f = uint2{(unsigned)w2[0], (unsigned)w2[1]};
+      const v4f* x = v + 4 * c;
+      const int y = __builtin_amdgcn_cvt_pk_fp8_f32(x[0][i] * inv, x[1][i] * inv, 0, false);
+      const int wd = __builtin_amdgcn_cvt_pk_fp8_f32(x[2][i] * inv, x[3][i] * inv, y, true);
+      *(int*)(frag + (rl / 16) * F8_LPITCH + (w >> 2) * 1024 + (c * 16 + rl % 16) * 16 + 4 * (w & 3)) = wd;
     }
     const int64_t r = rg * 256 + rl;
-    if (q == 0 && r < p.RB * 16) p.scales[(h * p.KB + kb) * p.RB * 16 + r] = 127 + e;
+    if (w == 0 && r < p.RB * 16) p.scales[(h * p.KB + kb) * p.RB * 16 + r] = 127 + e;
   }
   __syncthreads();
   // The block: fragments rg * 16 .. + 15 of (h, kb), 128 16-byte pieces each.
   const int64_t nfr = p.RB - rg * 16 < 16 ? p.RB - rg * 16 : 16;
   uint8_t* out = p.out + ((h * p.KB + kb) * p.RB + rg * 16) * F8_FRAG;
 #pragma unroll
-  for (int t = 0; t < 8; t++) {
-    const int idx = t * 256 + threadIdx.x;  // 16-byte piece of the 32 KiB block
+  for (int t = 0; t < 4; t++) {
+    const int idx = t * 512 + threadIdx.x;  // 16-byte piece of the 32 KiB block
     const int fr = idx >> 7, pc = idx & 127;
     if (fr < nfr) *(v4u*)(out + (int64_t)fr * F8_FRAG + pc * 16) = *(const v4u*)(frag + fr * F8_LPITCH + pc * 16);
   }
@@ -525,23 +522,45 @@ __global__ __launch_bounds__(G::NT) void k_gemm_fp8(F8GemmArgs g) {
                         p.cd_stride[3] % 4 == 0 && (uintptr_t)Dg % 16 == 0 &&
                         (!p.read_c || (p.cd_stride[0] == 1 && p.cd_stride[1] % 4 == 0 && (uintptr_t)Cg % 16 == 0));
     if (direct) {
+      // Staged column-major through LDS (pitch BM + 4: each 16-lane group's
+      // 16-byte stores hit distinct banks), CH columns at a time, then written
+      // as whole columns of the tile: 64 lanes cover 1 KiB of one column per
+      // store (16-byte stores straight from the accumulators covered 64-byte
+      // pieces of 16 columns: f32 transform step GEMM 175.5 against 171.7 us).
+      constexpr int LDM = BM + 4;
+      constexpr int CH = (size_t)BN * LDM * 4 <= G::LDS ? BN : BN / 2;
+      static_assert((size_t)CH * LDM * 4 <= G::LDS && BN % CH == 0 && G::WN <= CH && CH % G::WN == 0, "staging");
+      static_assert(BM * CH % (4 * G::NT) == 0, "whole vectors per thread");
+      float* Dt = (float*)smem;
       const float alr = (float)p.alpha[0], ber = (float)p.beta[0];
+#pragma unroll 1
+      for (int c = 0; c < BN / CH; c++) {
+        if (c > 0) __syncthreads();  // the previous columns have been written out
+        if (wn / CH == c) {
 #pragma unroll
-      for (int i = 0; i < G::MI; i++)
+          for (int i = 0; i < G::MI; i++)
 #pragma unroll
-        for (int j = 0; j < G::NI; j++) {
-          const int64_t m = m0 + wm + 16 * i + 4 * lk, n = n0 + wn + 16 * j + lr;
+            for (int j = 0; j < G::NI; j++)
+              *(v4f*)&Dt[(wn - c * CH + 16 * j + lr) * LDM + wm + 16 * i + 4 * lk] = acc[0][i][j];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int t = 0; t < BM * CH / (4 * G::NT); t++) {
+          const int e = threadIdx.x + t * G::NT;
+          const int ml = 4 * (e % (BM / 4)), nl = e / (BM / 4);
+          const int64_t m = m0 + ml, n = n0 + c * CH + nl;
           if (m >= p.M || n >= p.N) continue;
+          const v4f x = *(const v4f*)&Dt[nl * LDM + ml];
           const int64_t off = m + n * p.cd_stride[3], coff = m + n * p.cd_stride[1];
           if (p.M - m >= 4) {
-            v4f y = alr * acc[0][i][j];
+            v4f y = alr * x;
             if (p.read_c) y += ber * *(const v4f*)(Cg + coff);
             *(v4f*)(Dg + off) = y;
           } else {
-            for (int q = 0; q < p.M - m; q++)
-              Dg[off + q] = alr * acc[0][i][j][q] + (p.read_c ? ber * Cg[coff + q] : 0.0f);
+            for (int q = 0; q < p.M - m; q++) Dg[off + q] = alr * x[q] + (p.read_c ? ber * Cg[coff + q] : 0.0f);
           }
         }
+      }
       return;
     }
   }
@@ -604,7 +623,7 @@ hipError_t run(const ContractArgs& a, void* panels, hipStream_t s) {
     if (q->vec) hipLaunchKernelGGL((k_pack_fp8<CPLX, true>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     else if (q->rvec) {
       const int64_t g4 = q->H * q->KB * ((q->RB + 15) / 16);
-      hipLaunchKernelGGL(k_pack_fp8_rows4, dim3((unsigned)g4), dim3(256), 0, s, *q);
+      hipLaunchKernelGGL(k_pack_fp8_rows4, dim3((unsigned)g4), dim3(512), 0, s, *q);
     }
     else if (q->kmajor) hipLaunchKernelGGL((k_pack_fp8_lds<CPLX>), dim3((unsigned)grid), dim3(256), 0, s, *q);
     else hipLaunchKernelGGL((k_pack_fp8_rows<CPLX>), dim3((unsigned)grid), dim3(256), 0, s, *q);
