@@ -697,7 +697,7 @@ hipError_t launch_majors(const ContractArgs& a, hipStream_t s) {
 //    waves). c64 64x64: 8 waves of 32x16 (54.5 -> 58 against 4 of 32x32;
 //    3m without prefetch: 63 against 54).
 //  - c32 128x128: 16 waves of 32x32 (native 101 -> 109.5 and 3m 105 -> 119
-//    without prefetch, 1m 89 -> 97 with it).
+//    without prefetch, 1m 89 -> 97 with it), slabs of 32 k (Single::go).
 //  - f64 128x128 (grouped, M-major A, or any 16-byte run layout): 8 waves of
 //    64x32 without prefetch, at most 128 VGPRs so two workgroups share a CU; f64 64x64: 8 waves of
 //    32x16 (f64 24^6 31.2 -> 28.4 us against 4 of 32x32).
@@ -732,6 +732,29 @@ struct Single {
   hipStream_t s;
   template <int BM, int BN, int WM, int WN, bool PLR, int MINW>
   hipError_t go() const {
+    // c32 128x128: slabs of 32 k (its 8-byte elements fill the LDS of the c64
+    // tile's 16-k slabs): half the slab barriers of the f32 MFMA's shorter K
+    // steps (complex benchmark contraction: native 101.4 -> 110.9 TFLOP/s, 3m
+    // 111.3 -> 125.9). 1m only with A gathered along K (it spills with an
+    // M-major A). A K split into an odd number of 16-k slabs keeps 16.
+    constexpr bool k32 = CPLX && sizeof(T) == 4 && BM == 128 && BN == 128;
+    if constexpr (k32) {
+      if (a.ksplit <= 1 || a.kslabs % 2 == 0) {
+        if constexpr (METH == CPLX_1M) {
+          if (a.a_kmajor) {
+            if (a.ab_vec) {
+              if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, 32, WM, WN, true, true, METH, PLR, MINW, true>>(a, s);
+              return launch_cfg<DCfg<T, CPLX, BM, BN, 32, WM, WN, true, false, METH, PLR, MINW, true>>(a, s);
+            }
+            if (a.b_kmajor) return launch_cfg<DCfg<T, CPLX, BM, BN, 32, WM, WN, true, true, METH, PLR, MINW>>(a, s);
+            return launch_cfg<DCfg<T, CPLX, BM, BN, 32, WM, WN, true, false, METH, PLR, MINW>>(a, s);
+          }
+        } else {
+          if (a.ab_vec) return launch_majors<T, CPLX, BM, BN, 32, WM, WN, METH, PLR, MINW, true>(a, s);
+          return launch_majors<T, CPLX, BM, BN, 32, WM, WN, METH, PLR, MINW>(a, s);
+        }
+      }
+    }
     if constexpr (!CPLX || sizeof(T) == 4)
       if (a.ab_vec) return launch_majors<T, CPLX, BM, BN, 16, WM, WN, METH, PLR, MINW, true>(a, s);
     return launch_majors<T, CPLX, BM, BN, 16, WM, WN, METH, PLR, MINW>(a, s);
