@@ -437,13 +437,14 @@ Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
   p->m_ax = role_axes(r.m, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sa)); });
   p->n_ax = role_axes(r.n, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sb)); });
   // The operand's own contiguous axis leads its role (A's for M, B's for N)
-  // where D's contiguous axis is another one and the operand is at least as
-  // large as D (K >= N for A, K >= M for B): a tile's rows then read whole
-  // runs of A (B), and the rows of D they write are completed in L2 by the
-  // neighbouring tiles (TCCG ajbdc-ckbad-jk, all extents 24: A was read one
-  // element per cache line, f64 0.185 -> 0.126 ms). Where D is the larger
-  // tensor its writes keep the lead (CCSD(T) abcijk-ikma-mjbc: 0.196 against
-  // 0.48 ms with A's axis first).
+  // where D's contiguous axis is another one and the operand is at least a
+  // quarter of D's size (4 K >= N for A, 4 K >= M for B): a tile's rows then
+  // read whole runs of A (B), and the rows of D they write are completed in
+  // L2 by the neighbouring tiles (TCCG ajbdc-ckbad-jk, all extents 24: A was
+  // read one element per cache line, f64 0.185 -> 0.126 ms; ijkl-imjn-lnkm
+  // f64 25.4 -> 55.4 TFLOP/s). Where D is much the larger tensor its writes
+  // keep the lead (CCSD(T) abcijk-ikma-mjbc: 0.196 against 0.48 ms with A's
+  // axis first).
   auto lead = [](std::vector<Axis>& ax, int64_t Axis::*s) {
     for (size_t i = 1; i < ax.size(); i++)
       if (uabs(ax[i].*s) == 1) {
@@ -455,8 +456,8 @@ Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
   p->k_ax = role_axes(r.k, [](const Axis& x) { return std::make_pair(uabs(x.sa), uabs(x.sb)); });
   {
     const int64_t M = product(p->m_ax), N = product(p->n_ax), K = product(p->k_ax);
-    if (K >= N) lead(p->m_ax, &Axis::sa);
-    if (K >= M) lead(p->n_ax, &Axis::sb);
+    if (4 * K >= N) lead(p->m_ax, &Axis::sa);
+    if (4 * K >= M) lead(p->n_ax, &Axis::sb);
   }
   p->h_ax = role_axes(r.h, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sa)); });
   p->r_ax = role_axes(r.r, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sc)); });

@@ -417,16 +417,17 @@ def test_scatter_vectors_match_the_reference_examples():
 
 
 def test_operand_contiguous_axis_leads_its_role_when_the_operand_is_large():
-    """Two M axes, A contiguous along one and D along the other: with K >= N
-    (A at least as large as D) A's contiguous axis comes first in M, so a
-    tile's rows read runs of A (TCCG ajbdc-ckbad-jk-like); with K < N (D the
-    larger tensor) D's contiguous axis keeps the lead. N likewise with B."""
+    """Two M axes, A contiguous along one and D along the other: with 4 K >=
+    N (A at least a quarter of D's size) A's contiguous axis comes first in
+    M, so a tile's rows read runs of A (TCCG ajbdc-ckbad-jk-like); with D
+    much the larger tensor D's contiguous axis keeps the lead. N likewise
+    with B."""
     import tprims_amd as tp
     from harness import T, make_plan
     # D[a, c, n] = sum_k A[c, k, a] B[k, n], a and c M axes (A contiguous in c, D in a)
     ea, ec, ek = 6, 5, 8
     a = T([ec, ek, ea], [1, ec, ec * ek], [2, 3, 1])
-    for en, leads_with_c in ((4, True), (16, False)):
+    for en, leads_with_c in ((4, True), (32, True), (64, False)):
         b = T([ek, en], [1, ek], [3, 4])
         d = T([ea, ec, en], [1, ea, ea * ec], [1, 2, 4])
         sc = make_plan(tp.DType.F64, a, b, d).scatters()
@@ -439,7 +440,7 @@ def test_operand_contiguous_axis_leads_its_role_when_the_operand_is_large():
     # N: D[m, j, l] = sum_k A[m, k] B[l, k, j], B contiguous in l, D in j (after m)
     em, ej, el, ek = 4, 3, 5, 8
     b = T([el, ek, ej], [1, el, el * ek], [6, 3, 5])
-    for em, leads_with_l in ((4, True), (64, False)):
+    for em, leads_with_l in ((4, True), (32, True), (64, False)):
         a = T([em, ek], [1, em], [1, 3])
         d = T([em, ej, el], [1, em, em * ej], [1, 5, 6])
         sc = make_plan(tp.DType.F64, a, b, d).scatters()
