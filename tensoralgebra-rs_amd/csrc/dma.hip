@@ -161,6 +161,18 @@ struct DCfg {
   }
 };
 
+// The barrier that hands the staged partial tile from K group g - 1 to group
+// g: this wave's LDS stores complete before it. __syncthreads() alone
+// compiles here to a bare s_barrier at the top of the groups' loop (the
+// compiler counts on the LDS performing every wave's operations in one order
+// and drops the wait), and the next group then sometimes read the f32 column
+// staging without the previous group's 16-byte stores: wrong tiles in about 1
+// call in 100 of the f32 K-group tile at M = 72, N = 140608, K = 72.
+__device__ inline void handoff_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __syncthreads();
+}
+
 // One BM x BN output tile (tile row tm, column tn) of batch entry h, staged
 // through smem (C::LDS bytes): over all of K, written back to D; or, with
 // split K, over K range `split` (slabs [split * kslabs, + kslabs)), written
@@ -514,7 +526,7 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
         __syncthreads();  // the stages, or the previous column's staging, are free
 #pragma unroll 1
         for (int g = 0; g < KG; g++) {
-          if (g > 0) __syncthreads();
+          if (g > 0) handoff_barrier();
           if (kg == g && wn == c * WN) {
 #pragma unroll
             for (int i = 0; i < C::MR; i++)
@@ -560,7 +572,7 @@ __device__ inline void dma_tile(const ContractArgs& p, int64_t h, int tm, int tn
     if (c > 0) __syncthreads();  // the previous chunk's write-back has read the staging rows
 #pragma unroll 1
     for (int g = 0; g < KG; g++) {
-      if (g > 0) __syncthreads();  // group g - 1's values are in place
+      if (g > 0) handoff_barrier();  // group g - 1's values are in place
       const bool add = g > 0;
       if (kg == g && wm / ER == c) {
 #pragma unroll

@@ -168,3 +168,42 @@ def test_register_direct_tile_with_batch_entries_short_k_and_repeats(gpu):
         plan.execute_raw(1.5, ta.data_ptr(), tb.data_ptr(), 0.0, None, td.data_ptr())
         torch.cuda.synchronize()
         np.testing.assert_array_equal(td.cpu().numpy(), got)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", DTYPES, ids=lambda d: d.name)
+def test_kgroup_tile_hands_partial_sums_on_in_order(gpu, dtype):
+    """The K groups add their partial tiles through LDS in turn; each group's
+    stores must be complete before the next group reads them. M = 72, N =
+    140608, K = 72 (two slabs per group, every call's groups finishing close
+    together) over 300 calls: every result bitwise equal to the first and to
+    a float64 product within the storage tolerance. Without the wait before
+    the hand-off barrier the f32 tile gave wrong tiles in about 1 call in 100
+    here."""
+    import torch
+
+    M, Kx, N = 72, 72, 140608
+    a, b, d = _operands(M, Kx, N, True, True, conj=False)
+    plan = make_plan(dtype, a, b, d, config=tp.PlanConfig(tile=5))
+    assert ".kgroup4" in plan.report()["kernel"], plan.report()
+    dev = torch.device("cuda:0")
+    tdt = {tp.DType.F32: torch.float32, tp.DType.F64: torch.float64,
+           tp.DType.C32: torch.complex64, tp.DType.C64: torch.complex128}[dtype]
+    g = torch.Generator(device=dev).manual_seed(7)
+    A = torch.randn(a.storage(), dtype=tdt, device=dev, generator=g)
+    B = torch.randn(b.storage(), dtype=tdt, device=dev, generator=g)
+    # A[k, m] at k + K m, B[k, n] at k + K n, D[m, n] at m + M n.
+    want = (A.view(M, Kx).to(torch.complex128 if dtype.is_complex else torch.float64)
+            @ B.view(N, Kx).to(torch.complex128 if dtype.is_complex else torch.float64).T).T.reshape(-1)
+    D = torch.empty(d.storage(), dtype=tdt, device=dev)
+    first = None
+    for i in range(300):
+        D.fill_(float("nan"))
+        plan.execute_raw(1.0, A.data_ptr(), B.data_ptr(), 0.0, None, D.data_ptr())
+        torch.cuda.synchronize()
+        if first is None:
+            first = D.clone()
+            err = ((first.to(want.dtype) - want).abs().max() / want.abs().max()).item()
+            assert err < (1e-5 if dtype in (tp.DType.F32, tp.DType.C32) else 1e-13), err
+        else:
+            assert torch.equal(D, first), "call %d differs from the first" % i
