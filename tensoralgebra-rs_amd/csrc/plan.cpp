@@ -113,6 +113,31 @@ Axis scaled(const Axis& a, int64_t extent, int64_t by) {
   return {extent, a.sa * by, a.sb * by, a.sc * by, a.sd * by};
 }
 
+// Where A's and D's contiguous axes are different M axes, order M as D's axis
+// in blocks of up to 8, then A's in blocks of up to 8, then the rest of both,
+// then the other axes: a tile's rows cover 8 x 8 blocks that write runs of 8
+// elements of D and read runs of 8 of A, where either axis alone leading
+// leaves the other operand one element per cache line.
+void block_rows(std::vector<Axis>& m) {
+  int ia = -1, id = -1;
+  for (size_t i = 0; i < m.size(); i++) {
+    if (uabs(m[i].sa) == 1 && ia < 0) ia = (int)i;
+    if (uabs(m[i].sd) == 1 && id < 0) id = (int)i;
+  }
+  if (ia < 0 || id < 0 || ia == id) return;
+  const Axis xa = m[ia], xd = m[id];
+  const int64_t dLo = largest_divisor_le(xd.extent, 8), aLo = largest_divisor_le(xa.extent, 8);
+  if (dLo < 2 || aLo < 2) return;
+  std::vector<Axis> out;
+  push_nonunit(out, scaled(xd, dLo, 1));
+  push_nonunit(out, scaled(xa, aLo, 1));
+  push_nonunit(out, scaled(xd, xd.extent / dLo, dLo));
+  push_nonunit(out, scaled(xa, xa.extent / aLo, aLo));
+  for (size_t i = 0; i < m.size(); i++)
+    if ((int)i != ia && (int)i != id) out.push_back(m[i]);
+  m = out;
+}
+
 }  // namespace
 
 bool tile_allowed(int id, DType t, int prec, int method) {
@@ -160,6 +185,42 @@ int choose_tile(const Plan& p) {
     if (best == 0 || cost < best_cost * (1 - 1e-9)) {
       best_cost = cost;
       best = id;
+    }
+  }
+  // Where the MFMA estimate is not what bounds the time, the 64 x 64 tile
+  // (four waves per workgroup, several workgroups per CU) keeps more gathers
+  // in flight than one 128 x 128 (or 32 x 32) workgroup. Measured over the
+  // TCCG corpus at 64 MiB per tensor, every dtype, tiles forced one by one on
+  // one GPU; each rule applies with at least 1024 tiles of 64 x 64:
+  //  - K <= 32: a tile is little more than its gathers and its write-back
+  //    (CCSD(T) abcijk-ikmc-mjab, K = 24, f64 0.189 -> 0.147 ms; InTensLi
+  //    ajbdc-ckbad-jk c64 0.275 -> 0.176);
+  //  - K <= 128 instead of 128 x 64 or 128 x 128, except f64 with A
+  //    running along K, where the 16-byte-run 128 x 128 tile keeps the
+  //    lead (TCCG ajbc-ckba-jk, K = 52, f32 0.212 -> 0.135; f64 abjc-kbac-jk
+  //    0.120 on 128 x 128 against 0.149);
+  //  - complex instead of 128 x 128 unless A and B both run along K (c32
+  //    ijkl-imjn-nlmk 2.85 -> 2.39 ms, c64 5.49 -> 4.58; with both along K
+  //    c32 ijkl-minj-nlmk keeps 128 x 128: 4.29 against 5.21, and so does
+  //    the c64 headline product, whose A is a stride-2 view).
+  if (prec == PREC_STORAGE && best != TILE_64x64 && tile_allowed(TILE_64x64, p.prob.dtype, prec, p.cfg.complex_method)) {
+    const double tiles64 = (double)((p.M + 63) / 64) * (double)((p.N + 63) / 64) * (double)p.H;
+    // An operand runs along K where its smallest nonzero stride is a K
+    // axis's (a strided view such as the headline's stride-2 A included).
+    auto along_k = [&](int64_t Axis::*s, const std::vector<Axis>& rows) {
+      auto least = [&](const std::vector<Axis>& ax) {
+        uint64_t m = UINT64_MAX;
+        for (const Axis& x : ax)
+          if (x.*s != 0) m = std::min(m, uabs(x.*s));
+        return m;
+      };
+      return least(p.k_ax) < least(rows);
+    };
+    const bool a_k = along_k(&Axis::sa, p.m_ax), b_k = along_k(&Axis::sb, p.n_ax);
+    if (tiles64 >= 1024) {
+      if (p.K <= 32) return TILE_64x64;
+      if (p.K <= 128 && best != TILE_32x32 && !(p.prob.dtype == DType::F64 && a_k)) return TILE_64x64;
+      if (is_complex(p.prob.dtype) && best == TILE_128x128 && !(a_k && b_k)) return TILE_64x64;
     }
   }
   return best;
@@ -458,6 +519,11 @@ Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
     const int64_t M = product(p->m_ax), N = product(p->n_ax), K = product(p->k_ax);
     if (4 * K >= N) lead(p->m_ax, &Axis::sa);
     if (4 * K >= M) lead(p->n_ax, &Axis::sb);
+    // With a short K the reads of A and the writes of D are the work: block
+    // the rows so that both run (TCCG InTensLi ajbdc-ckbad-jk, extents 24,
+    // on the 64 x 64 tile: f64 0.104 -> 0.076 ms, f32 0.060 -> 0.047; abjc-
+    // cbka-kj, K = 72, f32 0.112 -> 0.080).
+    if (K <= 128) block_rows(p->m_ax);
   }
   p->h_ax = role_axes(r.h, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sa)); });
   p->r_ax = role_axes(r.r, [](const Axis& x) { return std::make_pair(uabs(x.sd), uabs(x.sc)); });

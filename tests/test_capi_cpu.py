@@ -368,17 +368,34 @@ def test_tile_choice_follows_the_busiest_cu_estimate():
     import tprims_amd as tp
     from harness import T, make_plan
 
-    def gemm(dt, m, n, k, h=1):
-        return make_plan(dt, T([m, k, h], [1, m, m * k], [0, 2, 3]), T([k, n, h], [1, k, k * n], [2, 1, 3]),
+    def gemm(dt, m, n, k, h=1, a_k=False):
+        a = T([m, k, h], [k, 1, m * k] if a_k else [1, m, m * k], [0, 2, 3])
+        return make_plan(dt, a, T([k, n, h], [1, k, k * n], [2, 1, 3]),
                          T([m, n, h], [1, m, m * n], [0, 1, 3]), config=tp.PlanConfig(split_k=1)).report()
 
     for dt in (tp.DType.F32, tp.DType.F64, tp.DType.C32, tp.DType.C64):
-        big = gemm(dt, 4096, 4096, 256)
+        # A contiguous along K for the complex dtypes: with an M-major A they
+        # take 64x64 (the complex rule below)
+        big = gemm(dt, 4096, 4096, 256, a_k=dt.is_complex)
         # f32, f64 and c32 gather 16-byte runs (column-major operands qualify)
         suffix = "128x128x16.dma" if dt == tp.DType.C64 else "128x128x16.dma16"
         assert big["tile"] == 4 and big["kernel"].endswith(suffix), big["kernel"]
-    # 512 x 512 x 16 batches fill the GPU with 128x128 tiles (the headline shape)
-    assert gemm(tp.DType.C64, 512, 512, 64, 16)["tile"] == 4
+        if dt.is_complex:
+            # A gathered along M: the 64x64 tile (TCCG ij-ik-kj c32 2904^3:
+            # 1.67 ms on 64x64 against 2.19 on 128x128)
+            assert gemm(dt, 4096, 4096, 256)["tile"] == 1
+    # 512 x 512 x 16 batches over K = 4096 with A and B contiguous along K
+    # fill the GPU with 128x128 tiles (the headline shape); with K = 64 the
+    # short-K rule takes 64x64
+    assert gemm(tp.DType.C64, 512, 512, 4096, 16, a_k=True)["tile"] == 4
+    assert gemm(tp.DType.C64, 512, 512, 64, 16, a_k=True)["tile"] == 1
+    # A a stride-2 view along K still runs along K (the headline's A)
+    for dt in (tp.DType.C32, tp.DType.C64):
+        m, k, n, h = 512, 4096, 512, 16
+        a2 = T([m, k, h], [2 * k, 2, 2 * m * k], [0, 2, 3])
+        rep = make_plan(dt, a2, T([k, n, h], [1, k, k * n], [2, 1, 3]), T([m, n, h], [1, m, m * n], [0, 1, 3]),
+                        config=tp.PlanConfig(split_k=1)).report()
+        assert rep["tile"] == 4, rep
     # one 64x64 output: a 64x64 tile on 4 SIMDs, not a 128x128 one
     small = gemm(tp.DType.C64, 64, 64, 64)
     assert small["tile"] == 1 and small["kernel"].endswith("64x64x16.dma"), small["kernel"]
@@ -424,10 +441,11 @@ def test_operand_contiguous_axis_leads_its_role_when_the_operand_is_large():
     with B."""
     import tprims_amd as tp
     from harness import T, make_plan
-    # D[a, c, n] = sum_k A[c, k, a] B[k, n], a and c M axes (A contiguous in c, D in a)
-    ea, ec, ek = 6, 5, 8
+    # D[a, c, n] = sum_k A[c, k, a] B[k, n], a and c M axes (A contiguous in
+    # c, D in a); K = 200 (up to K = 128 the rows are blocked instead)
+    ea, ec, ek = 6, 5, 200
     a = T([ec, ek, ea], [1, ec, ec * ek], [2, 3, 1])
-    for en, leads_with_c in ((4, True), (32, True), (64, False)):
+    for en, leads_with_c in ((4, True), (800, True), (1000, False)):
         b = T([ek, en], [1, ek], [3, 4])
         d = T([ea, ec, en], [1, ea, ea * ec], [1, 2, 4])
         sc = make_plan(tp.DType.F64, a, b, d).scatters()
@@ -445,6 +463,36 @@ def test_operand_contiguous_axis_leads_its_role_when_the_operand_is_large():
         d = T([em, ej, el], [1, em, em * ej], [1, 5, 6])
         sc = make_plan(tp.DType.F64, a, b, d).scatters()
         assert (sc["bn"][1] == 1) == leads_with_l, sc["bn"][:4]
+
+
+def test_rows_are_blocked_when_a_and_d_run_along_different_m_axes_and_k_is_short():
+    """D[a, c, n] = sum_k A[c, k, a] B[k, n] with extents 24 (TCCG InTensLi
+    ajbdc-ckbad-jk-like: A contiguous in c, D in a): with K <= 128 the M
+    order is a in blocks of 8, c in blocks of 8, then the rest of a and of c,
+    so a tile's rows write runs of 8 of D and read runs of 8 of A; the short
+    K also takes the 64x64 tile. K = 200 keeps one axis leading."""
+    import tprims_amd as tp
+    from harness import T, make_plan
+    e = 24
+    for ek, blocked in ((24, True), (128, True), (200, False)):
+        a = T([e, ek, e], [1, e, e * ek], [2, 3, 1])
+        b = T([ek, e], [1, ek], [3, 4])
+        d = T([e, e, e], [1, e, e * e], [1, 2, 4])
+        plan = make_plan(tp.DType.F64, a, b, d)
+        sc = plan.scatters()
+        if blocked:
+            assert sc["dm"][:9] == list(range(8)) + [e], sc["dm"][:9]
+            assert sc["am"][:2] == [0, e * ek] and sc["am"][8] == 1, sc["am"][:9]
+            # rows 64..: the second block of a (D offsets 8..15)
+            assert sc["dm"][64] == 8 and sc["am"][64] == 8 * e * ek
+        else:  # 4 K >= N: A's axis c leads (m = c + 24 a)
+            assert sc["am"][:e] == list(range(e)) and sc["dm"][:2] == [0, e], sc["am"][:4]
+    # the short-K tile rule: 24^3 rows x 24 columns of f32, K = 24
+    a = T([e, e, e, e], [1, e, e * e, e ** 3], [2, 3, 1, 5])
+    b = T([e, e], [1, e], [3, 4])
+    d = T([e, e, e, e], [1, e, e * e, e ** 3], [1, 2, 4, 5])
+    rep = make_plan(tp.DType.F32, a, b, d).report()
+    assert rep["tile"] == 1, rep
 
 
 def test_plan_scatters_are_the_reduced_matrix_view():
