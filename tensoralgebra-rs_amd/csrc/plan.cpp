@@ -187,42 +187,57 @@ int choose_tile(const Plan& p) {
       best = id;
     }
   }
-  // Where the MFMA estimate is not what bounds the time, the 64 x 64 tile
-  // (four waves per workgroup, several workgroups per CU) keeps more gathers
-  // in flight than one 128 x 128 (or 32 x 32) workgroup. Measured over the
-  // TCCG corpus at 64 MiB per tensor, every dtype, tiles forced one by one on
-  // one GPU; each rule applies with at least 1024 tiles of 64 x 64:
-  //  - K <= 32: a tile is little more than its gathers and its write-back
-  //    (CCSD(T) abcijk-ikmc-mjab, K = 24, f64 0.189 -> 0.147 ms; InTensLi
-  //    ajbdc-ckbad-jk c64 0.275 -> 0.176);
-  //  - K <= 128 instead of 128 x 64 or 128 x 128, except f64 with A
-  //    running along K, where the 16-byte-run 128 x 128 tile keeps the
-  //    lead (TCCG ajbc-ckba-jk, K = 52, f32 0.212 -> 0.135; f64 abjc-kbac-jk
-  //    0.120 on 128 x 128 against 0.149);
-  //  - complex instead of 128 x 128 unless A and B both run along K (c32
-  //    ijkl-imjn-nlmk 2.85 -> 2.39 ms, c64 5.49 -> 4.58; with both along K
-  //    c32 ijkl-minj-nlmk keeps 128 x 128: 4.29 against 5.21, and so does
-  //    the c64 headline product, whose A is a stride-2 view).
-  if (prec == PREC_STORAGE && best != TILE_64x64 && tile_allowed(TILE_64x64, p.prob.dtype, prec, p.cfg.complex_method)) {
-    const double tiles64 = (double)((p.M + 63) / 64) * (double)((p.N + 63) / 64) * (double)p.H;
-    // An operand runs along K where its smallest nonzero stride is a K
-    // axis's (a strided view such as the headline's stride-2 A included).
-    auto along_k = [&](int64_t Axis::*s, const std::vector<Axis>& rows) {
-      auto least = [&](const std::vector<Axis>& ax) {
-        uint64_t m = UINT64_MAX;
-        for (const Axis& x : ax)
-          if (x.*s != 0) m = std::min(m, uabs(x.*s));
-        return m;
-      };
-      return least(p.k_ax) < least(rows);
+  return best;
+}
+
+// Where the MFMA estimate is not what bounds the time, the 64 x 64 tile (four
+// to eight waves per workgroup, four workgroups per CU) keeps more gathers in
+// flight than one 128 x 128 (or 32 x 32) workgroup. Measured over the TCCG
+// corpus at 64 MiB per tensor, every dtype, tiles forced one by one on one
+// GPU, and on contiguous 6144 x 6144 products; each rule applies with at
+// least 1024 tiles of 64 x 64:
+//  - K <= 32 with operands not in 16-byte runs: a tile is little more than
+//    its gathers and its write-back (CCSD(T) abcijk-ikmc-mjab, K = 24, f64
+//    0.189 -> 0.147 ms; InTensLi ajbdc-ckbad-jk c64 0.275 -> 0.176). In
+//    16-byte runs the 128 x 128 tile stays (f64 6144 x 6144 x 24: 0.094
+//    against 0.139 on 64 x 64);
+//  - K <= 128 instead of 128 x 64 or 128 x 128, where the operands are not
+//    in 16-byte runs or M or N is at most 128, except f64 with A running
+//    along K (TCCG ajbc-ckba-jk, N = 72, K = 52, f32 0.212 -> 0.135; f64
+//    abjc-kbac-jk 0.120 on 128 x 128 against 0.149);
+//  - complex instead of 128 x 128 unless A and B both run along K (c32
+//    ijkl-imjn-nlmk 2.85 -> 2.39 ms, c64 5.49 -> 4.58; with both along K
+//    c32 ijkl-minj-nlmk keeps 128 x 128: 4.29 against 5.21, and so does the
+//    headline product, whose A is a stride-2 view: c32 100 against 74
+//    TFLOP/s); c32 in 16-byte runs takes 64 x 64 in any orientation
+//    (column-major 8192 x 8192 x 512 with A along K: 128 against 113
+//    TFLOP/s; 4096^3: 135 against 124).
+int gather_bound_tile(const Plan& p, int best) {
+  if (p.cfg.precision != PREC_STORAGE || best == TILE_64x64 ||
+      !tile_allowed(TILE_64x64, p.prob.dtype, PREC_STORAGE, p.cfg.complex_method))
+    return best;
+  const double tiles64 = (double)((p.M + 63) / 64) * (double)((p.N + 63) / 64) * (double)p.H;
+  if (tiles64 < 1024) return best;
+  // An operand runs along K where its smallest nonzero stride is a K axis's
+  // (a strided view such as the headline's stride-2 A included).
+  auto along_k = [&](int64_t Axis::*s, const std::vector<Axis>& rows) {
+    auto least = [&](const std::vector<Axis>& ax) {
+      uint64_t m = UINT64_MAX;
+      for (const Axis& x : ax)
+        if (x.*s != 0) m = std::min(m, uabs(x.*s));
+      return m;
     };
-    const bool a_k = along_k(&Axis::sa, p.m_ax), b_k = along_k(&Axis::sb, p.n_ax);
-    if (tiles64 >= 1024) {
-      if (p.K <= 32) return TILE_64x64;
-      if (p.K <= 128 && best != TILE_32x32 && !(p.prob.dtype == DType::F64 && a_k)) return TILE_64x64;
-      if (is_complex(p.prob.dtype) && best == TILE_128x128 && !(a_k && b_k)) return TILE_64x64;
-    }
-  }
+    return least(p.k_ax) < least(rows);
+  };
+  const bool a_k = along_k(&Axis::sa, p.m_ax), b_k = along_k(&Axis::sb, p.n_ax);
+  if (p.K <= 32 && !p.ab_vec && (best == TILE_32x32 || best == TILE_128x64 || best == TILE_128x128))
+    return TILE_64x64;
+  if (p.K <= 128 && (best == TILE_128x64 || best == TILE_128x128) && (!p.ab_vec || std::min(p.M, p.N) <= 128) &&
+      !(p.prob.dtype == DType::F64 && a_k))
+    return TILE_64x64;
+  if (is_complex(p.prob.dtype) && best == TILE_128x128 &&
+      (!(a_k && b_k) || (p.prob.dtype == DType::C32 && p.ab_vec)))
+    return TILE_64x64;
   return best;
 }
 
@@ -447,6 +462,7 @@ void configure(Plan* plan, const PlanConfig& cfg_in) {
     p.a_vec16 = operand(p.a_kmajor, p.am, p.M, p.ak, p.ha);
     p.ab_vec = p.a_vec16 && operand(p.b_kmajor, p.bn, p.N, p.bk, p.hb);
   }
+  if (cfg.tile == 0 && p.strategy == 0 && p.ksplit == 1) p.tile = gather_bound_tile(p, p.tile);
   // f64 in 16-byte runs takes the register-direct K-group tile (f64 24^6:
   // 16.4 against 20.6 us per call on the LDS-DMA one; 576 x 1152 x 576 27.8
   // against 34.2).
