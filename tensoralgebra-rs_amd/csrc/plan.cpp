@@ -533,8 +533,16 @@ Plan* make_plan(const Problem& pr, const PlanConfig& cfg) {
   p->k_ax = role_axes(r.k, [](const Axis& x) { return std::make_pair(uabs(x.sa), uabs(x.sb)); });
   {
     const int64_t M = product(p->m_ax), N = product(p->n_ax), K = product(p->k_ax);
-    if (4 * K >= N) lead(p->m_ax, &Axis::sa);
-    if (4 * K >= M) lead(p->n_ax, &Axis::sb);
+    // D written along one role leaves the other role's order free: there the
+    // operand's contiguous axis leads whatever the sizes.
+    auto has_unit = [](const std::vector<Axis>& ax, int64_t Axis::*s) {
+      for (const Axis& x : ax)
+        if (uabs(x.*s) == 1) return true;
+      return false;
+    };
+    const bool d_along_n = has_unit(p->n_ax, &Axis::sd), d_along_m = has_unit(p->m_ax, &Axis::sd);
+    if (4 * K >= N || d_along_n) lead(p->m_ax, &Axis::sa);
+    if (4 * K >= M || d_along_m) lead(p->n_ax, &Axis::sb);
     // With a short K the reads of A and the writes of D are the work: block
     // the rows so that both run (TCCG InTensLi ajbdc-ckbad-jk, extents 24,
     // on the 64 x 64 tile: f64 0.104 -> 0.076 ms, f32 0.060 -> 0.047; abjc-

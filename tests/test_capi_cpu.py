@@ -441,7 +441,8 @@ def test_operand_contiguous_axis_leads_its_role_when_the_operand_is_large():
     N (A at least a quarter of D's size) A's contiguous axis comes first in
     M, so a tile's rows read runs of A (TCCG ajbdc-ckbad-jk-like); with D
     much the larger tensor D's contiguous axis keeps the lead. N likewise
-    with B."""
+    with B; and where D runs along the other role the operand leads at any
+    size."""
     import tprims_amd as tp
     from harness import T, make_plan
     # D[a, c, n] = sum_k A[c, k, a] B[k, n], a and c M axes (A contiguous in
@@ -458,14 +459,23 @@ def test_operand_contiguous_axis_leads_its_role_when_the_operand_is_large():
         else:  # m = a + ea * c
             assert sc["dm"][:ea + 1] == list(range(ea)) + [ea], sc["dm"][:8]
             assert sc["am"][:2] == [0, ec * ek]
-    # N: D[m, j, l] = sum_k A[m, k] B[l, k, j], B contiguous in l, D in j (after m)
+    # N: D[m, j, l] = sum_k A[m, k] B[l, k, j], B contiguous in l, D in j
+    # (D's contiguous axis an N axis: the 4 K >= M rule decides)
     em, ej, el, ek = 4, 3, 5, 8
     b = T([el, ek, ej], [1, el, el * ek], [6, 3, 5])
     for em, leads_with_l in ((4, True), (32, True), (64, False)):
         a = T([em, ek], [1, em], [1, 3])
-        d = T([em, ej, el], [1, em, em * ej], [1, 5, 6])
+        d = T([em, ej, el], [ej, 1, em * ej], [1, 5, 6])
         sc = make_plan(tp.DType.F64, a, b, d).scatters()
         assert (sc["bn"][1] == 1) == leads_with_l, sc["bn"][:4]
+    # D contiguous in m (an M axis): its writes run along M whatever the N
+    # order, so B's contiguous axis leads N at any size (TCCG CCSD(T)
+    # abcijk-ikmc-mjab, A likewise with D along N: f32 1.21x)
+    for em in (4, 64, 4096):
+        a = T([em, ek], [1, em], [1, 3])
+        d = T([em, ej, el], [1, em, em * ej], [1, 5, 6])
+        sc = make_plan(tp.DType.F64, a, b, d).scatters()
+        assert sc["bn"][1] == 1, (em, sc["bn"][:4])
 
 
 def test_rows_are_blocked_when_a_and_d_run_along_different_m_axes_and_k_is_short():
