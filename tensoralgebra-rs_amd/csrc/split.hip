@@ -261,6 +261,10 @@ struct GCfg {
   static constexpr int IW = PW + AW;               // every copy instruction of a wave per K step
   static constexpr size_t A_RAW_BYTES = AF ? (size_t)KBLK * BM * 4 : 0;
   static constexpr int MI = WM / 16, NI = WN / 16;
+  // Every wave spans the tile's columns (waves stacked along M) on a split A:
+  // each A value of the tile is split by one wave only.
+  static constexpr bool WIDE = AF && WN == BN;
+  static constexpr int MINB = WIDE ? 512 / NT : 1;  // workgroups per CU the registers are bounded for
   static constexpr int LDD = BN + 1;
   static constexpr size_t STAGE_BYTES = A_RAW_BYTES + (size_t)FRAGS * FRAG * 2;
   static constexpr size_t LDS_MAIN = STAGES * STAGE_BYTES;
@@ -293,7 +297,7 @@ __device__ inline int araw_slot(int m, int k) {
 }
 
 template <class G>
-__global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
+__global__ __launch_bounds__(G::NT, G::MINB) void k_gemm(GemmArgs g) {
   const ContractArgs& p = g.c;
   __shared__ __attribute__((aligned(16))) char smem[G::LDS];
   const int64_t nwg = (int64_t)p.tiles_m * p.tiles_n * p.H;
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < 8; j++) v[j] = As[araw_slot<BM>(m, 8 * ag + j)];
   };
-  auto raw_split = [&](const float* v, int i) {
+  auto raw_split = [&](const float* v, int i, auto& dst) {
     v4u w[G::TERMS];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -394,7 +398,7 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
       for (int x = 0; x < G::TERMS; x++) w[x][q] = t[x];
     }
 #pragma unroll
-    for (int x = 0; x < G::TERMS; x++) a[0][x][i] = __builtin_bit_cast(v8bf, w[x]);
+    for (int x = 0; x < G::TERMS; x++) dst[0][x][i] = __builtin_bit_cast(v8bf, w[x]);
   };
   // B's fragments of a stage start after the A block (raw) or A's fragments.
   constexpr size_t B_OFF = G::AF ? G::A_RAW_BYTES : (size_t)G::A_FRAGS * FRAG * 2;
@@ -432,9 +436,57 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
 #pragma unroll
     for (int i = 0; i < G::MI; i++) {
       raw_read(smem, i, rv[0]);
-      raw_split(rv[0], i);
+      raw_split(rv[0], i, a);
     }
   }
+  if constexpr (G::WIDE) {
+    // Column block outer: column block j's B fragments of block kb + 1
+    // replace block kb's as soon as its MFMAs are issued (no second B
+    // register set). A's block kb + 1 is read up front and split one value
+    // pair per column block into `an`, so the split's VALU work is spread
+    // evenly between the MFMAs. Same stage protocol as the loop below.
+    static_assert(G::MI * 4 == G::NI, "one value pair of A per column block");
+    v8bf an[1][G::TERMS][G::MI];
+    for (int64_t kb = 0; kb < nk; kb++) {
+      __builtin_amdgcn_s_waitcnt(7 << 4);
+      __syncthreads();
+      if (kb + 2 < nk) issue(kb + 2, (int)(kb & 1));
+      const char* An = smem + (size_t)((kb + 1) & 1) * G::STAGE_BYTES;
+      const char* Bn = An + B_OFF;
+      float rw[G::MI][8];
+#pragma unroll
+      for (int i = 0; i < G::MI; i++) raw_read(An, i, rw[i]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < G::NI; j++) {
+#pragma unroll
+        for (int i = 0; i < G::MI; i++)
+#pragma unroll
+          for (int pp = 0; pp < NP; pp++)
+            acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][PA[pp]][i], b[0][PB[pp]][j], acc[0][i][j],
+                                                                    0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int x = 0; x < G::TERMS; x++) b[0][x][j] = frag(Bn, (wn >> 4) + j, 0, x);
+        {
+          const int i = j >> 2, q = j & 3;
+          uint32_t t[3];
+          split_pair<G::TERMS>(rw[i][2 * q], rw[i][2 * q + 1], t);
+#pragma unroll
+          for (int x = 0; x < G::TERMS; x++) {
+            v4u w = __builtin_bit_cast(v4u, an[0][x][i]);
+            w[q] = t[x];
+            an[0][x][i] = __builtin_bit_cast(v8bf, w);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int x = 0; x < G::TERMS; x++)
+#pragma unroll
+        for (int i = 0; i < G::MI; i++) a[0][x][i] = an[0][x][i];
+    }
+  } else
   for (int64_t kb = 0; kb < nk; kb++) {
     // vmcnt(0) and lgkmcnt(0): own copies landed, own LDS reads complete.
     __builtin_amdgcn_s_waitcnt(7 << 4);
@@ -470,7 +522,7 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
       if constexpr (G::AF) {
         // Row block i of block kb + 1: its values are read now and split
         // after the next row block's MFMAs (the LDS latency hides under them).
-        if (i > 0) raw_split(rv[(i - 1) & 1], i - 1);
+        if (i > 0) raw_split(rv[(i - 1) & 1], i - 1, a);
         raw_read(An, i, rv[i & 1]);
       } else {
 #pragma unroll
@@ -480,7 +532,7 @@ __global__ __launch_bounds__(G::NT) void k_gemm(GemmArgs g) {
       }
       __builtin_amdgcn_sched_barrier(0);
     }
-    if constexpr (G::AF) raw_split(rv[(G::MI - 1) & 1], G::MI - 1);
+    if constexpr (G::AF) raw_split(rv[(G::MI - 1) & 1], G::MI - 1, a);
 #pragma unroll
     for (int c = 0; c < G::COMPS; c++)
 #pragma unroll
@@ -611,7 +663,7 @@ hipError_t run(const ContractArgs& a, int tile_id, void* panels, hipStream_t s) 
   if constexpr (!CPLX) {
     if (fuse_a) {
       if (sh.bm == 256) launch_gemm<GCfg<CPLX, TERMS, 256, 128, 64, 64, 2, true>>(g, nwg, s);
-      else if (sh.bm == 128) launch_gemm<GCfg<CPLX, TERMS, 128, 128, 64, 64, 2, true>>(g, nwg, s);
+      else if (sh.bm == 128) launch_gemm<GCfg<CPLX, TERMS, 128, 128, 32, 128, 2, true>>(g, nwg, s);
       else launch_gemm<GCfg<CPLX, TERMS, 64, 64, 32, 32, 2, true>>(g, nwg, s);
       return hipGetLastError();
     }
