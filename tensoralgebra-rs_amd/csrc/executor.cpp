@@ -473,8 +473,8 @@ bool run_items_fused(Plan& p, Executor& ex, const double alpha[2], const double 
   a.hc = dev + 2 * n * H;
   a.hd = dev + 3 * n * H;
   a.h_affine = 0;
-  a.a_kvec = a.a_kvec && a16;
-  a.b_kvec = a.b_kvec && b16;
+  a.a_kvec = a16 ? a.a_kvec : 0;
+  a.b_kvec = b16 ? a.b_kvec : 0;
   a.ab_vec = a.ab_vec && a16 && b16;  // every item's origin, not only the first
   a.a_vec16 = a.a_vec16 && a16;
   // The items multiply the tile count: K is split only while the whole batch

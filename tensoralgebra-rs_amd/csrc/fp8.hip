@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_pack_fp8(F8PackArgs p) {
     const int64_t kt = k0 + kslot(lq, 4 * t);
     if constexpr (VEC) {
       const bool ok = rok && kt < p.K;  // K % 8 == 0: the 4 k in or out together
-      const v4f* src = (const v4f*)(X + (ok ? kt : 0));
+      const v4f* src = (const v4f*)(X + (!ok ? 0 : p.vec == 2 ? kt : p.kv[kt]));
       if constexpr (!CPLX) {
         const v4f x = src[0];
 #pragma unroll

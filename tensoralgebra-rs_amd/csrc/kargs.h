@@ -38,9 +38,12 @@ struct ContractArgs {
   int32_t d_mmajor;  // the write-back runs along M (1) or along N (0)
   int32_t read_c;    // beta != 0: C is read (never read otherwise)
   int32_t tiles_m, tiles_n;
-  // The K scatter of A (B) is the identity, K % 8 == 0 and every row and batch
-  // offset of the operand is 16-byte aligned: 8 consecutive k of one row load
-  // as whole 16-byte vectors (used by the split-BF16 packer).
+  // K % 8 == 0, every row and batch offset of A (B) is 16-byte aligned and
+  // the K scatter runs contiguously over each group of 8 k starting at a
+  // multiple of 8, from a 16-byte aligned offset: the 8 k of a group of one
+  // row load as whole 16-byte vectors (split-BF16 and FP8 packers). 2: the K
+  // scatter is the identity (offsets computed, no K table read); 1: group
+  // starts read from the K table; 0: neither.
   int32_t a_kvec, b_kvec;
   // Every gather offset of A and of B (rows plus k, in bytes from the batch
   // entry's base) is nonnegative and below 2^30: gathers use 32-bit buffer

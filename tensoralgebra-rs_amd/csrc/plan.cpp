@@ -354,15 +354,18 @@ void configure(Plan* plan, const PlanConfig& cfg_in) {
     if (v->empty()) v->push_back(0);
 
   auto kvec = [&](const std::vector<int64_t>& kv, const std::vector<int64_t>& rv, const std::vector<int64_t>& hv) {
-    if (p.K % 8 != 0) return false;
-    for (int64_t k = 0; k < p.K; k++)
-      if (kv[k] != k) return false;
+    if (p.K % 8 != 0) return 0;
     const int64_t q = 16 / es;  // elements per 16 bytes
     for (int64_t v : rv)
-      if (v % q != 0) return false;
+      if (v % q != 0) return 0;
     for (int64_t v : hv)
-      if (v % q != 0) return false;
-    return true;
+      if (v % q != 0) return 0;
+    bool identity = true;
+    for (int64_t k = 0; k < p.K; k++) {
+      identity = identity && kv[k] == k;
+      if (k % 8 == 0 ? kv[k] % q != 0 : kv[k] != kv[k - 1] + 1) return 0;
+    }
+    return identity ? 2 : 1;
   };
   p.a_kvec = kvec(p.ak, p.am, p.ha);
   p.b_kvec = kvec(p.bk, p.bn, p.hb);
@@ -675,8 +678,8 @@ ContractArgs Plan::args_with(const DeviceScatter& s, const void* A, const void* 
   a.b_kmajor = b_kmajor;
   a.d_mmajor = d_mmajor;
   a.read_c = read_c;
-  a.a_kvec = a_kvec && (uintptr_t)A % 16 == 0;
-  a.b_kvec = b_kvec && (uintptr_t)B % 16 == 0;
+  a.a_kvec = (uintptr_t)A % 16 == 0 ? a_kvec : 0;
+  a.b_kvec = (uintptr_t)B % 16 == 0 ? b_kvec : 0;
   a.off32 = off32;
   a.ab_vec = ab_vec && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0;
   a.a_vec16 = a_vec16 && (uintptr_t)A % 16 == 0;

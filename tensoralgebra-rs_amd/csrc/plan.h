@@ -46,7 +46,7 @@ struct Plan {
   bool a_kmajor = true, b_kmajor = true, d_mmajor = true;
   // Host half of ContractArgs::a_kvec / b_kvec (the pointer alignment is
   // checked per call).
-  bool a_kvec = false, b_kvec = false;
+  int a_kvec = 0, b_kvec = 0;
   bool off32 = false;  // ContractArgs::off32
   bool ab_vec = false;  // host half of ContractArgs::ab_vec (pointer alignment checked per call)
   bool a_vec16 = false;  // host half of ContractArgs::a_vec16

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs p) {
   float re[8], im[8];
   if constexpr (VEC) {
     const bool ok = rok && k0 < p.K;  // K % 8 == 0: all 8 k in range or none
-    const v4f* src = (const v4f*)(X + (ok ? k0 : 0));
+    const v4f* src = (const v4f*)(X + (!ok ? 0 : p.vec == 2 ? k0 : p.kv[k0]));
     if constexpr (!CPLX) {
       const v4f x0 = src[0], x1 = src[1];
 #pragma unroll

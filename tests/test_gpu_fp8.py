@@ -120,3 +120,23 @@ def test_fp8_is_refused_for_double():
     with pytest.raises(tp.TprimsError) as e:
         make_plan(tp.DType.C64, d, T([4, 4], [1, 4], [1, 2]), T([4, 4], [1, 4], [0, 2]), config=FP8)
     assert e.value.status == 15
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.C32], ids=lambda d: d.name)
+@pytest.mark.parametrize("L", [4, 8, 16, 24])
+def test_fp8_packer_k_runs(gpu, L, dtype):
+    """FP8 packing of operands whose K offsets are contiguous only within runs
+    of L, D[i,k] = sum_{j,l} A[l,i,j] B[l,k,j] (aligned groups of 4 k load as
+    16-byte vectors through the K table when L is a multiple of 8): exact on
+    representable inputs."""
+    I, Kx, J = 72, 40, 11
+    a = T([L, I, J], tp.col_major([L, I, J]), [0, 1, 2])
+    b = T([L, Kx, J], tp.col_major([L, Kx, J]), [0, 3, 2])
+    d = T([I, Kx], tp.col_major([I, Kx]), [1, 3])
+    av = grid_values(dtype, a.storage(), 4)
+    bv = grid_values(dtype, b.storage(), 5)
+    dv = np.zeros(d.storage(), NP[dtype])
+    got, want, plan = run(dtype, a, av, b, bv, d, dv, config=FP8)
+    assert "fp8_e4m3" in plan.report()["kernel"]
+    check(dtype, got, want, exact=True)

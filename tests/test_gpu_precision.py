@@ -117,3 +117,37 @@ def test_split_precision_is_refused_for_double():
         make_plan(tp.DType.F64, d, T([4, 4], [1, 4], [1, 2]), T([4, 4], [1, 4], [0, 2]),
                   config=tp.PlanConfig(precision=tp.Precision.SPLIT_BF16_3))
     assert e.value.status == 15
+
+
+def k_runs_layout(L, I, Kx, J):
+    """D[i,k] = sum_{j,l} A[l,i,j] B[l,k,j], column-major: both operands are
+    contiguous along l only, so their K offsets run in groups of L
+    consecutive elements with jumps between groups (16-byte K vectors for
+    the packers when L is a multiple of 8; L = 4 keeps the gather path)."""
+    a = T([L, I, J], tp.col_major([L, I, J]), [0, 1, 2])
+    b = T([L, Kx, J], tp.col_major([L, Kx, J]), [0, 3, 2])
+    d = T([I, Kx], tp.col_major([I, Kx]), [1, 3])
+    return a, b, d
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", SPLITS, ids=lambda p: p.name)
+@pytest.mark.parametrize("dtype", [tp.DType.F32, tp.DType.C32], ids=lambda d: d.name)
+@pytest.mark.parametrize("L", [4, 8, 16, 24])
+def test_split_packer_k_runs(gpu, L, dtype, precision):
+    """Split-BF16 packing of operands whose K offsets are contiguous only
+    within runs of L (aligned groups of 8 k load as 16-byte vectors through
+    the K table), against the oracle."""
+    I, Kx, J = 72, 40, 11
+    a, b, d = k_runs_layout(L, I, Kx, J)
+    rng = np.random.default_rng(L)
+    def vals(n):
+        v = rng.standard_normal(n)
+        if dtype.is_complex:
+            v = v + 1j * rng.standard_normal(n)
+        return v.astype(NP[dtype])
+    av, bv = vals(a.storage()), vals(b.storage())
+    dv = np.zeros(d.storage(), NP[dtype])
+    got, want, plan = run(dtype, a, av, b, bv, d, dv, config=tp.PlanConfig(precision=precision))
+    assert "split_bf16" in plan.report()["kernel"]
+    check(dtype, got, want, precision=precision)
