@@ -268,7 +268,10 @@ struct GCfg {
   static constexpr int LDD = BN + 1;
   static constexpr size_t STAGE_BYTES = A_RAW_BYTES + (size_t)FRAGS * FRAG * 2;
   static constexpr size_t LDS_MAIN = STAGES * STAGE_BYTES;
-  static constexpr size_t LDS_EPI = (size_t)COMPS * BM * LDD * 4;
+  // Epilogue staging: row-major re/im planes, or (real) the column-major
+  // tile of the 16-byte write-back of outputs contiguous along M.
+  static constexpr size_t LDS_ROWS = (size_t)COMPS * BM * LDD * 4, LDS_COLS = CPLX ? 0 : (size_t)BN * (BM + 4) * 4;
+  static constexpr size_t LDS_EPI = LDS_ROWS > LDS_COLS ? LDS_ROWS : LDS_COLS;
   static constexpr size_t LDS = LDS_MAIN > LDS_EPI ? LDS_MAIN : LDS_EPI;
   static_assert(FRAGS % NW == 0, "whole fragments per wave");
   static_assert(!AF || (!CPLX && A_INSTR % NW == 0 && A_CHUNKS_ROW % 16 == 0),
